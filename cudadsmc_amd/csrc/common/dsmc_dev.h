@@ -1,0 +1,165 @@
+// Device-side data layout of one DSMC instance and the per-thread RNG.
+//
+// HBM layout (all SoA, 4-byte lanes, 256-byte aligned allocations):
+//   particles  x,y,z,vx,vy,vz : float[cap], double-buffered (parity = step & 1)
+//   keys                      : uint32[cap] per buffer (cell of the particle,
+//                               0xFFFFFFFF = removed)
+//   count / cursor            : uint32[ncells]      next-step counting, scatter cursors
+//   cell_start                : uint32[ncells + 1]  offsets of the current order
+//   perm / perm_tmp           : uint32[cap]         sorted position -> storage index
+//   ntc                       : float2[ncells]      {(sigma c_r)max, remainder}
+//   samples                   : double[5][ncells]   {N, sum vx, sum vy, sum vz, sum |v|^2}
+//   RNG                       : uint32[20][n_streams] (19-word window + spare)
+// A step is scan(count) -> scatter -> cells(tile kernel) [-> inject].
+#pragma once
+#include <stdint.h>
+
+#include "detmath.h"
+
+namespace dsmc_dev {
+
+constexpr uint32_t kInvalidKey = 0xFFFFFFFFu;
+constexpr uint32_t kTileCells = 256;    // cells per tile (one lane per cell)
+constexpr uint32_t kTileThreads = 256;  // threads of the tile kernel
+constexpr uint32_t kLdsCap = 2048;      // particles staged in LDS per tile
+constexpr uint32_t kMtWords = 20;       // 19 window words + 1 spare (zero)
+constexpr int kMtN = 19, kMtM = 9;
+constexpr uint32_t kMtUpper = 0xFFFFFFFEu, kMtLower = 0x1u;
+constexpr int kShift0 = 12, kShiftB = 7, kShiftC = 15, kShift1 = 18;
+
+// Physical constants (reference values where the reference has them).
+constexpr float kBoltzRef = 1.3806e-23f;          // DSMC_BOLZ, include/dsmc.h:33
+constexpr double kBoltz = 1.380649e-23;           // CODATA, physics mode
+constexpr float kPiRef = 3.14159265358979f;       // DSMC_PI, include/dsmc.h:27
+constexpr double kBounceDelta = 0.001;            // BOUNCE_DELTA, dsmc_kernel.cu:53
+constexpr float kGeomEps = 0.0001f;               // EPS, dsmc_kernel.cu:1251
+constexpr float kModelWidth = 0.0001f;            // MODEL_WIDTH, dsmc_kernel.cu:1253
+constexpr int kCollisionLimit = 10;               // COLLISION_LIMIT, dsmc_kernel.cu:1250
+constexpr double kTriEpsilon = 0.000001;          // EPSILON, tri_intersect.cu:1
+constexpr int kPhysMaxBounces = 8;
+
+// Collision-grid cell flags.
+constexpr uint32_t kCgEmpty = 0, kCgSurface = 1, kCgFilled = 2;
+
+enum Mode : uint32_t { kModeReference = 0, kModePhysics = 1 };
+enum Face : uint32_t { kFaceSpecular = 0, kFacePeriodic = 1, kFaceFreestream = 2 };
+
+struct InstDesc {
+  // ---- simulation grid
+  float gmin[3], gmax[3], cell[3];
+  uint32_t dim[3];
+  uint32_t ncells;
+  uint32_t ntiles;
+  float cell_volume;
+  // ---- gas and NTC
+  float mass;
+  float sigma_hs;   // REFERENCE: (float)(pi d^2) evaluated in double
+  float vhs_k;      // PHYSICS: sigma*c_r = vhs_k * c_r^vhs_exp
+  float vhs_exp;    // 2 - 2 omega (1 for hard spheres)
+  float avg_conc;   // REFERENCE: global particles per cell
+  float fnum;
+  float dt;
+  float vm_wall;    // sqrt(2 k T_wall / m); <= 0 specular walls (PHYSICS)
+  float wall_temp;
+  uint32_t mode;
+  uint32_t periodic;      // REFERENCE box: 'x','y','z','n'
+  uint32_t face_bc[6];    // PHYSICS box faces
+  uint32_t has_geom;
+  // ---- freestream (PHYSICS injection)
+  float n_inf, T_inf, U[3], vm_inf;  // vm_inf = sqrt(2 k T_inf / m)
+  float inj_lambda[6];               // expected injections per face cell per step
+  uint32_t face_off[7];              // prefix of face cells per face (only freestream faces)
+  // ---- collision grid
+  float cg_min[3], cg_max[3], cg_cell[3];
+  uint32_t cg_dim[3];
+  uint32_t cg_ncells;
+  const uint32_t* cg_flag;
+  const uint32_t* cg_start;  // per cg cell: first entry (0xFFFFFFFF: none)
+  const uint32_t* cg_end;
+  const uint32_t* cg_tri;    // entries: triangle index
+  const float4* tv0;         // per triangle vertices and unit face normal (w unused)
+  const float4* tv1;
+  const float4* tv2;
+  const float4* tn;
+  // ---- particles [2] = double buffer
+  float* px[2];
+  float* py[2];
+  float* pz[2];
+  float* vx[2];
+  float* vy[2];
+  float* vz[2];
+  uint32_t* key[2];
+  uint32_t cap;
+  // ---- cells
+  uint32_t* count;
+  uint32_t* cursor;
+  uint32_t* cell_start;
+  uint32_t* perm;
+  uint32_t* perm_tmp;   // overflow tiles: sorted copy of the segment
+  uint8_t* owner_tmp;   // overflow tiles: cell of each position within the tile
+  float2* ntc;
+  float4* cstat;        // REFERENCE per-cell {selections, collisions, sum c_r, 0}
+  double* samp;         // [5][ncells]
+  float* fluid_frac;    // PHYSICS per-cell fluid volume fraction
+  // ---- scalars and per-workgroup accumulators
+  uint32_t* ctr;        // [0] stored particles, [1] sorted particles, [2] injected, [3] overflow tiles
+  double* wg_acc;       // [n_vwg][kAccSlots]
+  uint32_t* hits;       // per sorted particle wall hits (debug, may be null)
+  // ---- RNG
+  uint32_t* mt_state;   // [20][n_streams]
+  const uint32_t* mt_a;
+  const uint32_t* mt_b;
+  const uint32_t* mt_c;
+  uint32_t n_streams;
+  uint32_t n_vwg;       // virtual workgroups of the tile kernel = n_streams / 256
+  // ---- injection face cells
+  uint32_t n_face_cells;
+  uint32_t* face_count; // [n_face_cells + 1] counts then offsets
+};
+
+// Per virtual-workgroup accumulator slots (doubles).
+enum AccSlot {
+  kAccSel = 0, kAccColl, kAccCr, kAccFx, kAccFy, kAccFz, kAccHeat, kAccHits, kAccSlots
+};
+
+// ------------------------------------------------------------------- MT(19,9)
+// Sliding-window form of the DCMT generator: the state is the last 19
+// untempered words (x_k .. x_{k+18}); a draw computes
+//   x_{k+19} = x_{k+9} ^ twist(x_k, x_{k+1})
+// shifts the window by one and returns the tempered x_{k+19}. This is the
+// recurrence of the reference's incremental form (GetRandomIntegerFast,
+// kernels/mtwister.cu:70-100) and of Matsumoto-Nishimura's block form
+// (genrand_mt); keeping every index static lets the window live in VGPRs.
+struct Mt {
+  uint32_t st[kMtN];
+  uint32_t a, b, c;
+};
+
+DM_HD uint32_t mt_twist(uint32_t lo, uint32_t hi, uint32_t a) {
+  uint32_t x = (lo & kMtUpper) | (hi & kMtLower);
+  return (x >> 1) ^ ((x & 1u) ? a : 0u);
+}
+
+DM_HD uint32_t mt_next(Mt& m) {
+  uint32_t x = m.st[kMtM] ^ mt_twist(m.st[0], m.st[1], m.a);
+#pragma unroll
+  for (int k = 0; k < kMtN - 1; ++k) m.st[k] = m.st[k + 1];
+  m.st[kMtN - 1] = x;
+  x ^= (x >> kShift0);
+  x ^= (x << kShiftB) & m.b;
+  x ^= (x << kShiftC) & m.c;
+  x ^= (x >> kShift1);
+  return x;
+}
+
+// GetRandomFloatFast (kernels/mtwister.cu:102-105): ((float)u + 1) / 2^32, in (0, 1].
+DM_HD float mt_float(Mt& m) { return ((float)mt_next(m) + 1.0f) / 4294967296.0f; }
+
+// InitTwisterFast (kernels/mtwister.cu:107-116): Knuth's 1812433253 seeding.
+DM_HD void mt_seed(Mt& m, uint32_t seed) {
+  m.st[0] = seed;
+#pragma unroll
+  for (int i = 1; i < kMtN; ++i) m.st[i] = 1812433253u * (m.st[i - 1] ^ (m.st[i - 1] >> 30)) + (uint32_t)i;
+}
+
+}  // namespace dsmc_dev

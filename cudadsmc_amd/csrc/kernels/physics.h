@@ -1,0 +1,514 @@
+// Per-particle and per-cell physics of the DSMC step (device functions).
+//
+// REFERENCE mode reproduces the semantics of kernels/dsmc_kernel.cu of the
+// reference: cell keys (GetCellID3D :1701-1710), collision-grid lookup
+// (GetCollisionCellID :154-204), Moller-Trumbore (tri_intersect.cu:14-56),
+// first-hit specular walls (CollideWithGeometry :1256-1314), the move loop
+// (ProcessCollisionsDev :1596-1657), the box functors (:1316-1555) and the
+// hard-sphere NTC cell loop (ComputeBirdCollisionsDev :1002-1089).
+// PHYSICS mode adds VHS cross-sections, nearest-hit walls with diffuse
+// re-emission at the wall temperature and open freestream faces.
+//
+// FP policy: no contraction (-ffp-contract=off); the expression order of each
+// reference line is kept; transcendental functions come from detmath.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../common/dsmc_dev.h"
+
+namespace dsmc_dev {
+
+// ------------------------------------------------------------------ keys
+// GetCellID3D (dsmc_kernel.cu:1701-1710) with the GPU's saturating
+// float->uint conversion made explicit.
+__device__ __forceinline__ uint32_t cell_key(const InstDesc& I, float x, float y, float z) {
+  uint32_t ix = dm_f2u_sat((x - I.gmin[0]) / I.cell[0]);
+  uint32_t iy = dm_f2u_sat((y - I.gmin[1]) / I.cell[1]);
+  uint32_t iz = dm_f2u_sat((z - I.gmin[2]) / I.cell[2]);
+  ix = min(ix, I.dim[0] - 1u);
+  iy = min(iy, I.dim[1] - 1u);
+  iz = min(iz, I.dim[2] - 1u);
+  return iz * I.dim[0] * I.dim[1] + iy * I.dim[0] + ix;
+}
+
+// GetCollisionCellID (dsmc_kernel.cu:154-204).
+__device__ __forceinline__ uint32_t cg_cell(const InstDesc& I, float x, float y, float z) {
+  if (x < I.cg_min[0] || y < I.cg_min[1] || z < I.cg_min[2] || x > I.cg_max[0] || y > I.cg_max[1] ||
+      z > I.cg_max[2])
+    return kInvalidKey;
+  uint32_t ix = dm_f2u_sat((x - I.cg_min[0]) / I.cg_cell[0]);
+  uint32_t iy = dm_f2u_sat((y - I.cg_min[1]) / I.cg_cell[1]);
+  uint32_t iz = dm_f2u_sat((z - I.cg_min[2]) / I.cg_cell[2]);
+  ix = min(ix, I.cg_dim[0] - 1u);
+  iy = min(iy, I.cg_dim[1] - 1u);
+  iz = min(iz, I.cg_dim[2] - 1u);
+  return iz * I.cg_dim[0] * I.cg_dim[1] + iy * I.cg_dim[0] + ix;
+}
+
+// ----------------------------------------------------------- Moller-Trumbore
+// intersect_triangle (tri_intersect.cu:14-56): non-culling, EPSILON and
+// inv_det in double, the rest in float.
+__device__ __forceinline__ bool intersect_tri(float ox, float oy, float oz, float dx, float dy, float dz,
+                                              float4 a, float4 b, float4 c, float& t, float& u, float& v) {
+  float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
+  float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
+  float px = dy * e2z - dz * e2y;
+  float py = dz * e2x - dx * e2z;
+  float pz = dx * e2y - dy * e2x;
+  float det = e1x * px + e1y * py + e1z * pz;
+  if ((double)det > -kTriEpsilon && (double)det < kTriEpsilon) return false;
+  float inv_det = (float)(1.0 / (double)det);
+  float tx = ox - a.x, ty = oy - a.y, tz = oz - a.z;
+  u = (tx * px + ty * py + tz * pz) * inv_det;
+  if ((double)u < 0.0 || (double)u > 1.0) return false;
+  float qx = ty * e1z - tz * e1y;
+  float qy = tz * e1x - tx * e1z;
+  float qz = tx * e1y - ty * e1x;
+  v = (dx * qx + dy * qy + dz * qz) * inv_det;
+  if ((double)v < 0.0 || (double)(u + v) > 1.0) return false;
+  t = (e2x * qx + e2y * qy + e2z * qz) * inv_det;
+  return true;
+}
+
+// normalize (vectors.cu:24-31)
+__device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
+  float l = sqrtf(x * x + y * y + z * z);
+  if (l > 0.0f) {
+    x = x / l;
+    y = y / l;
+    z = z / l;
+  }
+}
+
+// CollideWithGeometry (dsmc_kernel.cu:1256-1314): first hit in list order.
+__device__ __forceinline__ bool ref_collide_geometry(const InstDesc& I, float& px, float& py, float& pz,
+                                                     float& vx, float& vy, float& vz, float& ox, float& oy,
+                                                     float& oz, uint32_t cell) {
+  uint32_t s = I.cg_start[cell], e = I.cg_end[cell];
+  float dx = vx, dy = vy, dz = vz;
+  normalize3(dx, dy, dz);
+  float ddx = px - ox, ddy = py - oy, ddz = pz - oz;
+  float dist = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);  // distance4 (vectors.cu:101-106)
+  if (dist < kGeomEps) return false;
+  for (uint32_t i = s; i < e; ++i) {
+    uint32_t tri = I.cg_tri[i];
+    float4 a = I.tv0[tri], b = I.tv1[tri], c = I.tv2[tri];
+    float t, u, v;
+    if (intersect_tri(ox, oy, oz, dx, dy, dz, a, b, c, t, u, v)) {
+      if (t >= 0.0f && t <= dist) {
+        float uv = 1.0f - u - v;
+        float4 n = I.tn[tri];
+        float wx = n.x * kModelWidth, wy = n.y * kModelWidth, wz = n.z * kModelWidth;
+        float hx = a.x * uv + b.x * u + c.x * v;
+        float hy = a.y * uv + b.y * u + c.y * v;
+        float hz = a.z * uv + b.z * u + c.z * v;
+        // reflect4 (vectors.cu:101-107... :96-100)
+        float dt = vx * n.x + vy * n.y + vz * n.z;
+        vx = vx - 2.0f * dt * n.x;
+        vy = vy - 2.0f * dt * n.y;
+        vz = vz - 2.0f * dt * n.z;
+        float rx = vx, ry = vy, rz = vz;
+        normalize3(rx, ry, rz);
+        float od = dist - t;
+        px = hx + rx * od;
+        py = hy + ry * od;
+        pz = hz + rz * od;
+        ox = hx + wx;
+        oy = hy + wy;
+        oz = hz + wz;
+        return true;
+      }
+    }
+  }
+  return false;
+}
+
+// Box functors of the reference (dsmc_kernel.cu:1316-1555). BOUNCE_DELTA is
+// a double literal, so the wrap/clamp expressions are evaluated in double.
+__device__ __forceinline__ float bd_add(float a, float b, float c) {
+  // a + BOUNCE_DELTA + b - c, evaluated left to right in double
+  return (float)((((double)a + kBounceDelta) + (double)b) - (double)c);
+}
+__device__ __forceinline__ float bd_sub(float a, float b, float c) {
+  return (float)((((double)a - kBounceDelta) + (double)b) - (double)c);
+}
+
+__device__ __forceinline__ void ref_box(const InstDesc& I, float& px, float& py, float& pz, float& vx,
+                                        float& vy, float& vz) {
+  const float* mn = I.gmin;
+  const float* mx = I.gmax;
+  const double D = kBounceDelta;
+  switch (I.periodic) {
+    case 'x':  // PeriodicXBoundaryCondition (:1392-1456)
+      if (px > mx[0]) px = bd_add(mn[0], px, mx[0]);
+      if (py > mx[1]) { py = (float)((double)mx[1] - D); vy *= -1.0f; }
+      if (px < mn[0]) px = bd_sub(mx[0], px, mn[0]);
+      if (py < mn[1]) { py = (float)((double)mn[1] + D); vy *= -1.0f; }
+      if (pz < mn[2]) { pz = (float)((double)mn[2] + D); vz *= -1.0f; }
+      if (pz > mx[2]) { pz = (float)((double)mx[2] - D); vz *= -1.0f; }
+      break;
+    case 'y':  // PeriodicYBoundaryCondition (:1458-1522)
+      if (py > mx[1]) py = bd_add(mn[1], py, mx[1]);
+      if (px > mx[0]) { px = (float)((double)mx[0] - D); vx *= -1.0f; }
+      if (py < mn[1]) py = bd_sub(mx[1], py, mn[1]);
+      if (px < mn[0]) { px = (float)((double)mn[0] + D); vx *= -1.0f; }
+      if (pz < mn[2]) { pz = (float)((double)mn[2] + D); vz *= -1.0f; }
+      if (pz > mx[2]) { pz = (float)((double)mx[2] - D); vz *= -1.0f; }
+      break;
+    case 'z':  // PeriodicZBoundaryCondition (:1316-1390)
+      if (pz > mx[2]) pz = bd_add(mn[2], pz, mx[2]);
+      if (py > mx[1]) { py = (float)((double)mx[1] - D); vy *= -1.0f; }
+      if (pz < mn[2]) pz = bd_sub(mx[2], pz, mn[2]);
+      if (py < mn[1]) { py = (float)((double)mn[1] + D); vy *= -1.0f; }
+      if (px < mn[0]) { px = (float)((double)mn[0] + D); vx *= -1.0f; }
+      if (px > mx[0]) { px = (float)((double)mx[0] - D); vx *= -1.0f; }
+      break;
+    default:  // NoPeriodicBoundaryCondition (:1524-1555)
+      if (py > mx[1]) { py = (float)((double)mx[1] - D); vy *= -1.0f; }
+      if (px > mx[0]) { px = (float)((double)mx[0] - D); vx *= -1.0f; }
+      if (py < mn[1]) { py = (float)((double)mn[1] + D); vy *= -1.0f; }
+      if (px < mn[0]) { px = (float)((double)mn[0] + D); vx *= -1.0f; }
+      if (pz < mn[2]) { pz = (float)((double)mn[2] + D); vz *= -1.0f; }
+      if (pz > mx[2]) { pz = (float)((double)mx[2] - D); vz *= -1.0f; }
+      break;
+  }
+}
+
+// ProcessCollisionsDev / IntegrateDev body (dsmc_kernel.cu:1557-1657).
+// Returns the number of wall reflections.
+__device__ __forceinline__ uint32_t move_reference(const InstDesc& I, float& px, float& py, float& pz,
+                                                   float& vx, float& vy, float& vz, float dt) {
+  float ox = px, oy = py, oz = pz;
+  float nx = ox + vx * dt, ny = oy + vy * dt, nz = oz + vz * dt;
+  uint32_t hits = 0;
+  if (I.has_geom) {
+    bool coll = false;  // not reset inside the loop, as in the reference
+    int i = 0;
+    do {
+      uint32_t st = cg_cell(I, ox, oy, oz);
+      uint32_t en = cg_cell(I, nx, ny, nz);
+      if (st != kInvalidKey) {
+        if (I.cg_flag[st] != 0u) coll = ref_collide_geometry(I, nx, ny, nz, vx, vy, vz, ox, oy, oz, st);
+      }
+      if (!coll && st != en && en != kInvalidKey) {
+        if (I.cg_flag[en] != 0u) coll = ref_collide_geometry(I, nx, ny, nz, vx, vy, vz, ox, oy, oz, en);
+      }
+      if (coll) ++hits;
+    } while (coll && i++ < kCollisionLimit);
+  }
+  ref_box(I, nx, ny, nz, vx, vy, vz);
+  px = nx;
+  py = ny;
+  pz = nz;
+  return hits;
+}
+
+// ---------------------------------------------------------------- physics
+// Nearest front-facing triangle hit along o + s*d, 0 <= s <= dist, over the
+// collision-grid cells the segment crosses (3-D DDA). Ties on s go to the
+// lower triangle index.
+struct DdaAxis {
+  int c, step;
+  float tmax, tdel;
+};
+
+__device__ __forceinline__ bool dda_clip(float o, float d, float lo, float hi, float& t0, float& t1) {
+  if (d == 0.0f) return !(o < lo || o > hi);
+  float ta = (lo - o) / d;
+  float tb = (hi - o) / d;
+  if (ta > tb) { float tmp = ta; ta = tb; tb = tmp; }
+  t0 = ta > t0 ? ta : t0;
+  t1 = tb < t1 ? tb : t1;
+  return true;
+}
+
+__device__ __forceinline__ DdaAxis dda_axis(float o, float d, float t0, float lo, float cell, uint32_t dim) {
+  DdaAxis r;
+  float p = o + d * t0;
+  float f = (p - lo) / cell;
+  int ci = (int)dm_floorf(f);
+  ci = ci < 0 ? 0 : (ci >= (int)dim ? (int)dim - 1 : ci);
+  r.c = ci;
+  if (d > 0.0f) {
+    r.step = 1;
+    float bnd = lo + (float)(ci + 1) * cell;
+    r.tmax = (bnd - o) / d;
+    r.tdel = cell / d;
+  } else if (d < 0.0f) {
+    r.step = -1;
+    float bnd = lo + (float)ci * cell;
+    r.tmax = (bnd - o) / d;
+    r.tdel = -cell / d;
+  } else {
+    r.step = 0;
+    r.tmax = 3.0e38f;
+    r.tdel = 3.0e38f;
+  }
+  return r;
+}
+
+__device__ __forceinline__ bool phys_nearest_hit(const InstDesc& I, float ox, float oy, float oz, float dx,
+                                                 float dy, float dz, float dist, uint32_t skip,
+                                                 float& s_hit, uint32_t& tri_hit) {
+  float t0 = 0.0f, t1 = dist;
+  if (!dda_clip(ox, dx, I.cg_min[0], I.cg_max[0], t0, t1)) return false;
+  if (!dda_clip(oy, dy, I.cg_min[1], I.cg_max[1], t0, t1)) return false;
+  if (!dda_clip(oz, dz, I.cg_min[2], I.cg_max[2], t0, t1)) return false;
+  if (t0 > t1) return false;
+  DdaAxis ax = dda_axis(ox, dx, t0, I.cg_min[0], I.cg_cell[0], I.cg_dim[0]);
+  DdaAxis ay = dda_axis(oy, dy, t0, I.cg_min[1], I.cg_cell[1], I.cg_dim[1]);
+  DdaAxis az = dda_axis(oz, dz, t0, I.cg_min[2], I.cg_cell[2], I.cg_dim[2]);
+  bool found = false;
+  float best = 0.0f;
+  uint32_t best_tri = 0;
+  const int max_iter = (int)(I.cg_dim[0] + I.cg_dim[1] + I.cg_dim[2]) + 3;
+  for (int it = 0; it < max_iter; ++it) {
+    uint32_t cell = (uint32_t)az.c * I.cg_dim[0] * I.cg_dim[1] + (uint32_t)ay.c * I.cg_dim[0] + (uint32_t)ax.c;
+    if (I.cg_flag[cell] == kCgSurface) {
+      uint32_t s = I.cg_start[cell], e = I.cg_end[cell];
+      for (uint32_t i = s; i < e; ++i) {
+        uint32_t tri = I.cg_tri[i];
+        if (tri == skip) continue;
+        float4 n = I.tn[tri];
+        if (!(dx * n.x + dy * n.y + dz * n.z < 0.0f)) continue;
+        float t, u, v;
+        if (intersect_tri(ox, oy, oz, dx, dy, dz, I.tv0[tri], I.tv1[tri], I.tv2[tri], t, u, v)) {
+          if (t >= 0.0f && t <= dist) {
+            if (!found || t < best || (t == best && tri < best_tri)) {
+              found = true;
+              best = t;
+              best_tri = tri;
+            }
+          }
+        }
+      }
+    }
+    bool xmin = ax.tmax < ay.tmax ? ax.tmax < az.tmax : false;
+    bool ymin = !xmin && (ay.tmax < az.tmax);
+    float texit = xmin ? ax.tmax : (ymin ? ay.tmax : az.tmax);
+    if (found && best <= texit) break;
+    if (texit > t1) break;
+    if (xmin) {
+      ax.c += ax.step;
+      if (ax.c < 0 || ax.c >= (int)I.cg_dim[0]) break;
+      ax.tmax = ax.tmax + ax.tdel;
+    } else if (ymin) {
+      ay.c += ay.step;
+      if (ay.c < 0 || ay.c >= (int)I.cg_dim[1]) break;
+      ay.tmax = ay.tmax + ay.tdel;
+    } else {
+      az.c += az.step;
+      if (az.c < 0 || az.c >= (int)I.cg_dim[2]) break;
+      az.tmax = az.tmax + az.tdel;
+    }
+  }
+  s_hit = best;
+  tri_hit = best_tri;
+  return found;
+}
+
+// Orthonormal tangent pair of a unit normal.
+__device__ __forceinline__ void tangents(float nx, float ny, float nz, float* t1, float* t2) {
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  float fx = nx < 0.0f ? -nx : nx, fy = ny < 0.0f ? -ny : ny, fz = nz < 0.0f ? -nz : nz;
+  if (fx <= fy && fx <= fz) ax = 1.0f;
+  else if (fy <= fz) ay = 1.0f;
+  else az = 1.0f;
+  float cx = ny * az - nz * ay, cy = nz * ax - nx * az, cz = nx * ay - ny * ax;
+  normalize3(cx, cy, cz);
+  t1[0] = cx; t1[1] = cy; t1[2] = cz;
+  t2[0] = ny * cz - nz * cy;
+  t2[1] = nz * cx - nx * cz;
+  t2[2] = nx * cy - ny * cx;
+}
+
+// One axis of the PHYSICS box: returns false when the particle leaves
+// through a freestream face.
+__device__ __forceinline__ bool phys_face(float& q, float& v, float lo, float hi, uint32_t f_lo, uint32_t f_hi) {
+  if (q < lo) {
+    if (f_lo == kFaceFreestream) return false;
+    if (f_lo == kFacePeriodic) q = q + (hi - lo);
+    else { q = lo + (lo - q); v = -v; }
+  } else if (q > hi) {
+    if (f_hi == kFaceFreestream) return false;
+    if (f_hi == kFacePeriodic) q = q - (hi - lo);
+    else { q = hi - (q - hi); v = -v; }
+  }
+  q = q < lo ? lo : (q > hi ? hi : q);
+  return true;
+}
+
+struct WallAcc {
+  double fx, fy, fz, heat;
+};
+
+// PHYSICS move: free flight with nearest-hit walls (diffuse at wall_temp or
+// specular), then the box faces. Returns the new cell key or kInvalidKey
+// when the particle left through a freestream face.
+__device__ __forceinline__ uint32_t move_physics(const InstDesc& I, float& px, float& py, float& pz, float& vx,
+                                                 float& vy, float& vz, float dt, Mt& rng, uint32_t& hits,
+                                                 WallAcc& acc) {
+  hits = 0;
+  float tl = dt;
+  if (I.has_geom) {
+    uint32_t skip = kInvalidKey;
+    for (int b = 0; b < kPhysMaxBounces; ++b) {
+      float sp = sqrtf(vx * vx + vy * vy + vz * vz);
+      float dist = sp * tl;
+      if (!(dist > 0.0f)) break;
+      float dx = vx / sp, dy = vy / sp, dz = vz / sp;
+      float s;
+      uint32_t tri;
+      if (!phys_nearest_hit(I, px, py, pz, dx, dy, dz, dist, skip, s, tri)) break;
+      float4 n = I.tn[tri];
+      float hx = px + dx * s, hy = py + dy * s, hz = pz + dz * s;
+      float ux, uy, uz;
+      if (I.vm_wall > 0.0f) {
+        float r1 = mt_float(rng), r2 = mt_float(rng), r3 = mt_float(rng);
+        float vn = I.vm_wall * sqrtf(-dm_logf(r1));
+        float rt = I.vm_wall * sqrtf(-dm_logf(r2));
+        float sn, cs;
+        dm_sincosf(6.28318530717958647692f * r3, &sn, &cs);
+        float a1 = rt * cs, a2 = rt * sn;
+        float t1[3], t2[3];
+        tangents(n.x, n.y, n.z, t1, t2);
+        ux = n.x * vn + t1[0] * a1 + t2[0] * a2;
+        uy = n.y * vn + t1[1] * a1 + t2[1] * a2;
+        uz = n.z * vn + t1[2] * a1 + t2[2] * a2;
+      } else {
+        float dd = vx * n.x + vy * n.y + vz * n.z;
+        ux = vx - 2.0f * dd * n.x;
+        uy = vy - 2.0f * dd * n.y;
+        uz = vz - 2.0f * dd * n.z;
+      }
+      double m = (double)I.mass;
+      acc.fx += m * ((double)vx - (double)ux);
+      acc.fy += m * ((double)vy - (double)uy);
+      acc.fz += m * ((double)vz - (double)uz);
+      acc.heat += 0.5 * m * (((double)vx * vx + (double)vy * vy + (double)vz * vz) -
+                             ((double)ux * ux + (double)uy * uy + (double)uz * uz));
+      ++hits;
+      tl = tl - s / sp;
+      const float off = 1.0e-6f;
+      px = hx + n.x * off;
+      py = hy + n.y * off;
+      pz = hz + n.z * off;
+      vx = ux;
+      vy = uy;
+      vz = uz;
+      skip = tri;
+    }
+  }
+  if (tl > 0.0f) {
+    px = px + vx * tl;
+    py = py + vy * tl;
+    pz = pz + vz * tl;
+  }
+  if (!phys_face(px, vx, I.gmin[0], I.gmax[0], I.face_bc[0], I.face_bc[1])) return kInvalidKey;
+  if (!phys_face(py, vy, I.gmin[1], I.gmax[1], I.face_bc[2], I.face_bc[3])) return kInvalidKey;
+  if (!phys_face(pz, vz, I.gmin[2], I.gmax[2], I.face_bc[4], I.face_bc[5])) return kInvalidKey;
+  return cell_key(I, px, py, pz);
+}
+
+// ------------------------------------------------------------------- NTC
+// Isotropic post-collision velocities (dsmc_kernel.cu:1057-1078).
+__device__ __forceinline__ void scatter_pair(float* vx, float* vy, float* vz, uint32_t i, uint32_t j, float mag,
+                                             Mt& rng) {
+  float v1x = vx[i], v1y = vy[i], v1z = vz[i];
+  float v2x = vx[j], v2y = vy[j], v2z = vz[j];
+  float b = 2.0f * mt_float(rng) - 1.0f;
+  float a = sqrtf(1.0f - b * b);
+  float c = 6.28318548202514648438f * mt_float(rng);  // 2*DSMC_PI in float
+  float sn, cs;
+  dm_sincosf(c, &sn, &cs);
+  float ax = b * mag / 2.0f, ay = a * mag * cs / 2.0f, az = a * mag * sn / 2.0f;
+  vx[i] = (v1x + v2x) / 2.0f + ax;
+  vy[i] = (v1y + v2y) / 2.0f + ay;
+  vz[i] = (v1z + v2z) / 2.0f + az;
+  vx[j] = (v1x + v2x) / 2.0f - ax;
+  vy[j] = (v1y + v2y) / 2.0f - ay;
+  vz[j] = (v1z + v2z) / 2.0f - az;
+}
+
+// ComputeBirdCollisionsDev cell loop (dsmc_kernel.cu:1002-1089) on the n
+// particles of one cell (velocity arrays already offset to the cell).
+__device__ __forceinline__ void ntc_reference(const InstDesc& I, float* vx, float* vy, float* vz, uint32_t n,
+                                              float2& nt, float4& st, Mt& rng) {
+  int part = (int)n;
+  if (part < 2) return;
+  float fpc = (float)((0.5 * (double)part * (double)I.avg_conc * (double)I.fnum * (double)nt.x *
+                       (double)I.dt) / (double)I.cell_volume + (double)nt.y);
+  uint32_t pairs = dm_f2u_sat(fpc);
+  st.x += (float)pairs;
+  nt.y = fpc - (float)pairs;
+  for (uint32_t k = 0; k < pairs; ++k) {
+    uint32_t i = mt_next(rng) % (uint32_t)part;
+    uint32_t j = mt_next(rng) % (uint32_t)part;
+    float dx = vx[i] - vx[j], dy = vy[i] - vy[j], dz = vz[i] - vz[j];
+    float mag = sqrtf(dx * dx + dy * dy + dz * dz);
+    float v = mag * I.sigma_hs;
+    if (v / nt.x > mt_float(rng)) {
+      st.y += 1.0f;
+      st.z += mag;
+      scatter_pair(vx, vy, vz, i, j, mag, rng);
+    }
+    if (v > nt.x) nt.x = v;
+  }
+}
+
+struct CollAcc {
+  double sel, coll, cr;
+};
+
+// NTC with N(N-1)/2 candidate pairs, distinct partners and the VHS
+// cross-section sigma*c_r = vhs_k * c_r^(2-2 omega) (Bird 1994, sec. 4.3).
+__device__ __forceinline__ void ntc_physics(const InstDesc& I, float* vx, float* vy, float* vz, uint32_t n,
+                                            float vol, float2& nt, Mt& rng, CollAcc& acc) {
+  if (n < 2u) return;
+  float fpc = (float)(0.5 * (double)n * (double)(n - 1u) * (double)I.fnum * (double)nt.x * (double)I.dt /
+                          (double)vol + (double)nt.y);
+  uint32_t pairs = dm_f2u_sat(fpc);
+  nt.y = fpc - (float)pairs;
+  acc.sel += (double)pairs;
+  for (uint32_t k = 0; k < pairs; ++k) {
+    uint32_t i = mt_next(rng) % n;
+    uint32_t j = mt_next(rng) % (n - 1u);
+    if (j >= i) ++j;
+    float dx = vx[i] - vx[j], dy = vy[i] - vy[j], dz = vz[i] - vz[j];
+    float mag = sqrtf(dx * dx + dy * dy + dz * dz);
+    float sc = (I.vhs_exp == 1.0f) ? I.vhs_k * mag : (mag > 0.0f ? I.vhs_k * dm_powf(mag, I.vhs_exp) : 0.0f);
+    if (sc > nt.x) nt.x = sc;
+    if (sc / nt.x > mt_float(rng)) {
+      acc.coll += 1.0;
+      acc.cr += (double)mag;
+      scatter_pair(vx, vy, vz, i, j, mag, rng);
+    }
+  }
+}
+
+// ---------------------------------------------------------- body interior
+// Point inside the body: collision-grid cell filled, or, in a surface cell,
+// behind the triangle of that cell whose centroid is nearest.
+__device__ __forceinline__ bool inside_body(const InstDesc& I, float x, float y, float z) {
+  if (!I.has_geom) return false;
+  uint32_t c = cg_cell(I, x, y, z);
+  if (c == kInvalidKey) return false;
+  uint32_t f = I.cg_flag[c];
+  if (f == kCgFilled) return true;
+  if (f != kCgSurface) return false;
+  float best = 3.0e38f, sd = 1.0f;
+  for (uint32_t i = I.cg_start[c]; i < I.cg_end[c]; ++i) {
+    uint32_t tri = I.cg_tri[i];
+    float4 a = I.tv0[tri], b = I.tv1[tri], cc = I.tv2[tri], n = I.tn[tri];
+    float cx = (a.x + b.x + cc.x) / 3.0f, cy = (a.y + b.y + cc.y) / 3.0f, cz = (a.z + b.z + cc.z) / 3.0f;
+    float rx = x - cx, ry = y - cy, rz = z - cz;
+    float d2 = rx * rx + ry * ry + rz * rz;
+    if (d2 < best) {
+      best = d2;
+      sd = rx * n.x + ry * n.y + rz * n.z;
+    }
+  }
+  return sd < 0.0f;
+}
+
+}  // namespace dsmc_dev

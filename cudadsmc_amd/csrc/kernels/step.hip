@@ -1,0 +1,914 @@
+// HIP kernels of the DSMC step for gfx950 (CDNA4, wave64).
+//
+// Step = scan(count) -> scatter -> tile [-> inject], replacing the
+// reference's CalculateCellIDs -> thrust::sort_by_key -> cudaMemset ->
+// FindCellBoundaries -> ComputeBird -> IntegrateAndProcessCollisions ->
+// SampleCellStatistics sequence (include/gpu_dsmc.inl:221-274).
+//
+//  * Keys of the next step are computed by the move itself and counted per
+//    cell with wave-aggregated atomics, so the "sort" is a counting sort:
+//    an exclusive scan of the counts gives the cell offsets
+//    (cellStartEnd of FindCellBoundariesDev, dsmc_kernel.cu:1734-1787) and a
+//    scatter with per-cell cursors gives the permutation.
+//  * The tile kernel owns 256 consecutive cells: it restores the stable
+//    order inside each cell segment (ascending storage index, i.e. the
+//    permutation of a stable sort_by_key, dsmc_kernel.cu:2048), gathers the
+//    particles (the REORDER gather, :1777-1786), runs the NTC loop one lane
+//    per cell out of LDS (:1002-1089), moves each particle with walls and
+//    box faces (:1596-1657), writes the next key and samples per cell
+//    (:856-890).
+//  * RNG: stream s = 256 * virtual_workgroup + lane; the tile kernel runs a
+//    fixed grid of n_streams/256 workgroups, each walking tiles vwg,
+//    vwg + n_vwg, ... so cell c always draws from stream c mod n_streams.
+#include <hip/hip_runtime.h>
+
+#include "launch.h"
+#include "physics.h"
+
+using namespace dsmc_dev;
+
+namespace {
+
+constexpr uint32_t kScanThreads = 256;
+constexpr uint32_t kScanItems = 16;
+constexpr uint32_t kScanTile = kScanThreads * kScanItems;  // 4096
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ void mt_load(const InstDesc& I, uint32_t s, Mt& m) {
+#pragma unroll
+  for (int w = 0; w < kMtN; ++w) m.st[w] = I.mt_state[(uint32_t)w * I.n_streams + s];
+  m.a = I.mt_a[s];
+  m.b = I.mt_b[s];
+  m.c = I.mt_c[s];
+}
+
+__device__ __forceinline__ void mt_store(const InstDesc& I, uint32_t s, const Mt& m) {
+#pragma unroll
+  for (int w = 0; w < kMtN; ++w) I.mt_state[(uint32_t)w * I.n_streams + s] = m.st[w];
+}
+
+// Wave-aggregated atomicAdd(&cnt[key], 1) for keys of consecutive lanes:
+// one atomic per run of equal keys. Invalid keys are skipped.
+__device__ __forceinline__ void count_keys(uint32_t* cnt, uint32_t key) {
+  uint32_t lane = lane_id();
+  uint32_t prev = __shfl_up(key, 1);
+  bool head = (lane == 0) || (key != prev);
+  uint64_t heads = __ballot(head);
+  if (head && key != kInvalidKey) {
+    uint64_t above = heads & ~((2ull << lane) - 1ull);
+    if (lane == 63) above = 0;
+    uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
+    atomicAdd(&cnt[key], next - lane);
+  }
+}
+
+}  // namespace
+
+// ======================================================================= RNG
+// InitMersenneTwistersDev (dsmc_kernel.cu:548-562): seed every stream.
+__global__ void k_mt_seed(uint32_t* state, const uint32_t* seeds, uint32_t n_streams) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_streams) return;
+  Mt m;
+  mt_seed(m, seeds[s]);
+#pragma unroll
+  for (int w = 0; w < kMtN; ++w) state[(uint32_t)w * n_streams + s] = m.st[w];
+  state[(uint32_t)kMtN * n_streams + s] = 0u;
+}
+
+// Stream outputs for the RNG parity tests (GenerateRandomSamplesUint4,
+// mtwister.cu:207-230, with one stream per thread).
+__global__ void k_mt_generate(const uint32_t* a, const uint32_t* b, const uint32_t* c, const uint32_t* seeds,
+                              uint32_t n_streams, uint32_t n_draws, uint32_t* out) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_streams) return;
+  Mt m;
+  m.a = a[s];
+  m.b = b[s];
+  m.c = c[s];
+  mt_seed(m, seeds[s]);
+  for (uint32_t d = 0; d < n_draws; ++d) out[(size_t)d * n_streams + s] = mt_next(m);
+}
+
+// ====================================================================== keys
+// CalculateCellIDsDev (dsmc_kernel.cu:1712-1724) on SoA positions.
+__global__ void k_cell_ids(InstDesc I, const float* x, const float* y, const float* z, uint32_t n, uint32_t* keys) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = cell_key(I, x[i], y[i], z[i]);
+}
+
+// Keys and counts of a freshly loaded state (storage order, buffer `par`).
+__global__ void k_keys_count(const InstDesc* __restrict__ Ip, uint32_t par, uint32_t n) {
+  const InstDesc& I = *Ip;
+  uint32_t base = blockIdx.x * blockDim.x;
+  uint32_t i = base + threadIdx.x;
+  uint32_t key = kInvalidKey;
+  if (i < n) {
+    key = cell_key(I, I.px[par][i], I.py[par][i], I.pz[par][i]);
+    I.key[par][i] = key;
+  }
+  count_keys(I.count, key);
+}
+
+// Plain counting of arbitrary keys (sort operator).
+__global__ void k_count_keys(const uint32_t* keys, uint32_t n, uint32_t* cnt) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t key = i < n ? keys[i] : kInvalidKey;
+  count_keys(cnt, key);
+}
+
+// ====================================================================== scan
+// Exclusive scan of uint32 counts in three passes (tile sums, scan of the
+// tile sums, tile scan). Optionally copies the offsets into a cursor array,
+// clears the counts and stores the total.
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* in, uint32_t n, uint32_t* tile_sums) {
+  __shared__ uint32_t red[kScanThreads / 64];
+  uint32_t base = blockIdx.x * kScanTile;
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    uint32_t i = base + k * kScanThreads + threadIdx.x;
+    if (i < n) sum += in[i];
+  }
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off);
+  if (lane_id() == 0) red[threadIdx.x / 64] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < kScanThreads / 64; ++w) t += red[w];
+    tile_sums[blockIdx.x] = t;
+  }
+}
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uint32_t* sh) {
+  // sh: kScanThreads/64 + 1 words
+  uint32_t lane = lane_id(), wid = threadIdx.x / 64;
+  uint32_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(x, off);
+    if (lane >= (uint32_t)off) x += y;
+  }
+  if (lane == 63) sh[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+      uint32_t t = sh[w];
+      sh[w] = run;
+      run += t;
+    }
+    sh[kScanThreads / 64] = run;
+  }
+  __syncthreads();
+  uint32_t r = sh[wid] + x - v;
+  *total = sh[kScanThreads / 64];
+  __syncthreads();
+  return r;
+}
+
+// Scans tile_sums in place (n_tiles <= kScanTile) and writes the grand total.
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(uint32_t* tile_sums, uint32_t n_tiles, uint32_t* total_out) {
+  __shared__ uint32_t sh[kScanThreads / 64 + 1];
+  uint32_t v[kScanItems];
+  uint32_t local = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    uint32_t i = threadIdx.x * kScanItems + k;
+    v[k] = i < n_tiles ? tile_sums[i] : 0u;
+    local += v[k];
+  }
+  uint32_t total;
+  uint32_t off = block_exclusive_scan(local, &total, sh);
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    uint32_t i = threadIdx.x * kScanItems + k;
+    if (i < n_tiles) tile_sums[i] = off;
+    off += v[k];
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply(uint32_t* in, uint32_t n, const uint32_t* tile_sums,
+                                                            uint32_t* out, uint32_t* cursor, uint32_t clear_in,
+                                                            const uint32_t* total) {
+  __shared__ uint32_t sh[kScanThreads / 64 + 1];
+  uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t local = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    uint32_t i = base + k;
+    v[k] = i < n ? in[i] : 0u;
+    local += v[k];
+  }
+  uint32_t tot;
+  uint32_t off = block_exclusive_scan(local, &tot, sh) + tile_sums[blockIdx.x];
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    uint32_t i = base + k;
+    if (i < n) {
+      out[i] = off;
+      if (cursor) cursor[i] = off;
+      if (clear_in) in[i] = 0u;
+    }
+    off += v[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = *total;
+}
+
+// =================================================================== scatter
+// perm[cursor[key[i]]++] = i with one atomic per run of equal keys in a wave.
+__global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip, uint32_t par) {
+  const InstDesc& I = *Ip;
+  const uint32_t n = I.ctr[0];
+  const uint32_t* keys = I.key[par];
+  const uint32_t lane = lane_id();
+  for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t key = i < n ? keys[i] : kInvalidKey;
+    uint32_t prev = __shfl_up(key, 1);
+    bool head = (lane == 0) || (key != prev);
+    uint64_t heads = __ballot(head);
+    uint32_t slot = 0;
+    if (head && key != kInvalidKey) {
+      uint64_t above = heads & ~((2ull << lane) - 1ull);
+      if (lane == 63) above = 0;
+      uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
+      slot = atomicAdd(&I.cursor[key], next - lane);
+    }
+    uint64_t mine = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+    uint32_t hl = 63u - (uint32_t)__clzll((long long)mine);
+    uint32_t hslot = __shfl(slot, (int)hl);
+    uint32_t dst = hslot + (lane - hl);
+    if (key != kInvalidKey && dst < I.cap) I.perm[dst] = i;
+  }
+}
+
+// Generic scatter for the sort operator.
+__global__ void k_scatter_keys(const uint32_t* keys, uint32_t n, uint32_t* cursor, uint32_t* perm) {
+  const uint32_t lane = lane_id();
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t key = i < n ? keys[i] : kInvalidKey;
+  uint32_t prev = __shfl_up(key, 1);
+  bool head = (lane == 0) || (key != prev);
+  uint64_t heads = __ballot(head);
+  uint32_t slot = 0;
+  if (head && key != kInvalidKey) {
+    uint64_t above = heads & ~((2ull << lane) - 1ull);
+    if (lane == 63) above = 0;
+    uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
+    slot = atomicAdd(&cursor[key], next - lane);
+  }
+  uint64_t mine = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+  uint32_t hl = 63u - (uint32_t)__clzll((long long)mine);
+  uint32_t hslot = __shfl(slot, (int)hl);
+  if (key != kInvalidKey) perm[hslot + (lane - hl)] = i;
+}
+
+// Stable order inside each cell segment (insertion sort, one lane per cell).
+__global__ void k_fix_segments(const uint32_t* start, uint32_t n_cells, uint32_t* perm) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  uint32_t a = start[c], b = start[c + 1];
+  for (uint32_t i = a + 1; i < b; ++i) {
+    uint32_t v = perm[i];
+    uint32_t j = i;
+    while (j > a && perm[j - 1] > v) {
+      perm[j] = perm[j - 1];
+      --j;
+    }
+    perm[j] = v;
+  }
+}
+
+// ====================================================================== tile
+using dsmc_launch::kFlagSample;
+using dsmc_launch::kFlagKeepPerm;
+using dsmc_launch::kFlagHits;
+
+template <uint32_t MODE, bool LDS>
+__device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint32_t flags, uint32_t c0, uint32_t nc,
+                                          uint32_t p0, uint32_t np, const uint32_t* s_start, uint32_t* s_raw,
+                                          uint32_t* s_perm, uint8_t* s_owner, float* s_vx, float* s_vy,
+                                          float* s_vz, Mt& rng, CollAcc& cacc, WallAcc& wacc) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t opar = par ^ 1u;
+  const float* __restrict__ ipx = I.px[par];
+  const float* __restrict__ ipy = I.py[par];
+  const float* __restrict__ ipz = I.pz[par];
+  const float* __restrict__ ivx = I.vx[par];
+  const float* __restrict__ ivy = I.vy[par];
+  const float* __restrict__ ivz = I.vz[par];
+  float* opx = I.px[opar] + p0;
+  float* opy = I.py[opar] + p0;
+  float* opz = I.pz[opar] + p0;
+  float* ovx = I.vx[opar] + p0;
+  float* ovy = I.vy[opar] + p0;
+  float* ovz = I.vz[opar] + p0;
+  uint32_t* okey = I.key[opar] + p0;
+  uint32_t* raw = LDS ? s_raw : (I.perm + p0);
+  uint32_t* srt = LDS ? s_perm : (I.perm_tmp + p0);
+  uint8_t* own = LDS ? s_owner : (I.owner_tmp + p0);
+  float* vx = LDS ? s_vx : ovx;
+  float* vy = LDS ? s_vy : ovy;
+  float* vz = LDS ? s_vz : ovz;
+
+  // (1) raw permutation of the tile and the owning cell of each position
+  if (LDS) {
+    for (uint32_t q = tid; q < np; q += kTileThreads) raw[q] = I.perm[p0 + q];
+  }
+  if (tid < nc) {
+    uint32_t a = s_start[tid] - p0, b = s_start[tid + 1] - p0;
+    for (uint32_t q = a; q < b; ++q) own[q] = (uint8_t)tid;
+  }
+  __syncthreads();
+  // (2) rank of each entry among its cell's entries -> stable order
+  for (uint32_t q = tid; q < np; q += kTileThreads) {
+    uint32_t c = own[q];
+    uint32_t a = s_start[c] - p0, b = s_start[c + 1] - p0;
+    uint32_t v = raw[q];
+    uint32_t r = 0;
+    for (uint32_t j = a; j < b; ++j) r += raw[j] < v ? 1u : 0u;
+    srt[a + r] = v;
+  }
+  __syncthreads();
+  // (3) gather velocities in sorted order
+  for (uint32_t q = tid; q < np; q += kTileThreads) {
+    uint32_t src = srt[q];
+    vx[q] = ivx[src];
+    vy[q] = ivy[src];
+    vz[q] = ivz[src];
+  }
+  __syncthreads();
+  // (4) NTC collisions, one lane per cell
+  if (tid < nc) {
+    uint32_t c = c0 + tid;
+    uint32_t a = s_start[tid] - p0, b = s_start[tid + 1] - p0;
+    uint32_t n = b - a;
+    if (n >= 2u) {
+      float2 nt = I.ntc[c];
+      if (MODE == kModeReference) {
+        float4 st = I.cstat[c];
+        ntc_reference(I, vx + a, vy + a, vz + a, n, nt, st, rng);
+        I.cstat[c] = st;
+      } else {
+        float vol = I.cell_volume * (I.fluid_frac ? I.fluid_frac[c] : 1.0f);
+        ntc_physics(I, vx + a, vy + a, vz + a, n, vol, nt, rng, cacc);
+      }
+      I.ntc[c] = nt;
+    }
+  }
+  __syncthreads();
+  // (5) move, walls, box faces, next key; rounds keep waves converged for
+  // the aggregated counting
+  const uint32_t rounds = (np + kTileThreads - 1u) / kTileThreads;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    uint32_t q = r * kTileThreads + tid;
+    uint32_t key = kInvalidKey;
+    if (q < np) {
+      uint32_t src = srt[q];
+      float x = ipx[src], y = ipy[src], z = ipz[src];
+      float ux = vx[q], uy = vy[q], uz = vz[q];
+      uint32_t hits;
+      if (MODE == kModeReference) {
+        hits = move_reference(I, x, y, z, ux, uy, uz, I.dt);
+        key = cell_key(I, x, y, z);
+      } else {
+        key = move_physics(I, x, y, z, ux, uy, uz, I.dt, rng, hits, wacc);
+      }
+      opx[q] = x;
+      opy[q] = y;
+      opz[q] = z;
+      okey[q] = key;
+      if (LDS) {
+        ovx[q] = ux;
+        ovy[q] = uy;
+        ovz[q] = uz;
+      }
+      vx[q] = ux;
+      vy[q] = uy;
+      vz[q] = uz;
+      if (flags & kFlagHits) I.hits[p0 + q] = hits;
+      if (flags & kFlagKeepPerm) I.perm[p0 + q] = src;
+    }
+    count_keys(I.count, key);
+  }
+  // (6) sampling of the post-move velocities, one lane per cell
+  if (flags & kFlagSample) {
+    __syncthreads();
+    if (tid < nc) {
+      uint32_t c = c0 + tid;
+      uint32_t a = s_start[tid] - p0, b = s_start[tid + 1] - p0;
+      if (b > a) {
+        if (MODE == kModeReference) {
+          // SampleCellStatisticsDev (dsmc_kernel.cu:856-890), FP32 sums
+          float* S = reinterpret_cast<float*>(I.samp);
+          const uint32_t C = I.ncells;
+          float sn = S[c], sx = S[C + c], sy = S[2 * C + c], sz = S[3 * C + c], s2 = S[4 * C + c];
+          for (uint32_t q = a; q < b; ++q) {
+            float ux = vx[q], uy = vy[q], uz = vz[q];
+            sn += 1.0f;
+            sx += ux;
+            sy += uy;
+            sz += uz;
+            s2 += ux * ux + uy * uy + uz * uz;
+          }
+          S[c] = sn; S[C + c] = sx; S[2 * C + c] = sy; S[3 * C + c] = sz; S[4 * C + c] = s2;
+        } else {
+          double* S = I.samp;
+          const uint32_t C = I.ncells;
+          double sn = S[c], sx = S[C + c], sy = S[2 * C + c], sz = S[3 * C + c], s2 = S[4 * C + c];
+          for (uint32_t q = a; q < b; ++q) {
+            double ux = vx[q], uy = vy[q], uz = vz[q];
+            sn += 1.0;
+            sx += ux;
+            sy += uy;
+            sz += uz;
+            s2 += ux * ux + uy * uy + uz * uz;
+          }
+          S[c] = sn; S[C + c] = sx; S[2 * C + c] = sy; S[3 * C + c] = sz; S[4 * C + c] = s2;
+        }
+      }
+    }
+  }
+}
+
+template <uint32_t MODE>
+__global__ void __launch_bounds__(kTileThreads) k_tile(const InstDesc* __restrict__ Ip, uint32_t par, uint32_t flags) {
+  const InstDesc& I = *Ip;
+  __shared__ uint32_t s_start[kTileCells + 1];
+  __shared__ uint32_t s_raw[kLdsCap];
+  __shared__ uint32_t s_perm[kLdsCap];
+  __shared__ uint8_t s_owner[kLdsCap];
+  __shared__ float s_vx[kLdsCap];
+  __shared__ float s_vy[kLdsCap];
+  __shared__ float s_vz[kLdsCap];
+  __shared__ double s_red[kTileThreads / 64][kAccSlots];
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t vw = blockIdx.x;
+  const uint32_t stream = vw * kTileThreads + tid;
+  Mt rng;
+  mt_load(I, stream, rng);
+  CollAcc cacc = {0.0, 0.0, 0.0};
+  WallAcc wacc = {0.0, 0.0, 0.0, 0.0};
+  double hits_acc = 0.0;
+  (void)hits_acc;
+  if (vw == 0 && tid == 0) I.ctr[0] = I.ctr[1];
+
+  for (uint32_t tile = vw; tile < I.ntiles; tile += gridDim.x) {
+    const uint32_t c0 = tile * kTileCells;
+    const uint32_t nc = min(kTileCells, I.ncells - c0);
+    s_start[tid] = I.cell_start[c0 + min(tid, nc)];
+    if (tid == 0) s_start[nc] = I.cell_start[c0 + nc];
+    __syncthreads();
+    const uint32_t p0 = s_start[0];
+    const uint32_t np = s_start[nc] - p0;
+    if (np <= kLdsCap)
+      tile_body<MODE, true>(I, par, flags, c0, nc, p0, np, s_start, s_raw, s_perm, s_owner, s_vx, s_vy, s_vz, rng,
+                            cacc, wacc);
+    else
+      tile_body<MODE, false>(I, par, flags, c0, nc, p0, np, s_start, s_raw, s_perm, s_owner, s_vx, s_vy, s_vz,
+                             rng, cacc, wacc);
+    __syncthreads();
+  }
+  mt_store(I, stream, rng);
+
+  // per-workgroup accumulators (deterministic: fixed reduction order)
+  double v[kAccSlots] = {cacc.sel, cacc.coll, cacc.cr, wacc.fx, wacc.fy, wacc.fz, wacc.heat, 0.0};
+#pragma unroll
+  for (int k = 0; k < kAccSlots; ++k) {
+    double x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+    if (lane_id() == 0) s_red[tid / 64][k] = x;
+  }
+  __syncthreads();
+  if (tid < kAccSlots) {
+    double t = 0.0;
+    for (uint32_t w = 0; w < kTileThreads / 64; ++w) t += s_red[w][tid];
+    I.wg_acc[vw * kAccSlots + tid] += t;
+  }
+}
+
+template __global__ void k_tile<kModeReference>(const InstDesc*, uint32_t, uint32_t);
+template __global__ void k_tile<kModePhysics>(const InstDesc*, uint32_t, uint32_t);
+
+// =================================================================== loading
+// GenerateInitialVelocitiesDevBlocks / GenerateUniformVelocitiesDevBlocks
+// (dsmc_kernel.cu:574-680): one thread per (x, y, i) column, i < ppc, walking
+// z; stream X*Y*i + y*X + x; particle index = cell*ppc + i. Cells with a
+// flagged collision-grid corner get parked particles (pos = vel = 0).
+__device__ __forceinline__ bool geometry_in_cell(const InstDesc& I, float cx, float cy, float cz, float sx, float sy,
+                                                 float sz) {
+  // IsGeometryInCell (dsmc_kernel.cu:479-546), corners in the reference's order
+  float px = cx, py = cy, pz = cz;
+  uint32_t c;
+#define DSMC_CORNER()                                                  \
+  if ((c = cg_cell(I, px, py, pz)) != kInvalidKey && I.cg_flag[c] != 0u) \
+    return true;
+  DSMC_CORNER();
+  px = px + sx;
+  DSMC_CORNER();
+  py = py + sy;
+  DSMC_CORNER();
+  pz = pz + sz;
+  DSMC_CORNER();
+  px = cx; py = cy; pz = cz;
+  py = py + sy;
+  DSMC_CORNER();
+  pz = pz + sz;
+  DSMC_CORNER();
+  py = cy;
+  DSMC_CORNER();
+  px = px + sx;
+  DSMC_CORNER();
+#undef DSMC_CORNER
+  return false;
+}
+
+__global__ void k_load_reference(const InstDesc* __restrict__ Ip, uint32_t par, uint32_t ppc, float sux, float suy,
+                                 float suz, float temp) {
+  const InstDesc& I = *Ip;
+  const uint32_t X = I.dim[0], Y = I.dim[1], Z = I.dim[2];
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= X * Y * ppc) return;
+  const uint32_t i = t / (X * Y);
+  const uint32_t xy = t - i * X * Y;
+  const uint32_t cy = xy / X, cx = xy - cy * X;
+  const uint32_t stream = t;  // = X*Y*i + y*X + x
+  Mt rng;
+  mt_load(I, stream, rng);
+  float cellx = I.gmin[0] + (float)cx * I.cell[0];
+  float celly = I.gmin[1] + (float)cy * I.cell[1];
+  const float twopi = 2.0f * kPiRef;
+  for (uint32_t z = 0; z < Z; ++z) {
+    uint32_t idx = (X * Y * z + cy * X + cx) * ppc + i;
+    float cellz = I.gmin[2] + (float)z * I.cell[2];
+    bool geom = I.has_geom && geometry_in_cell(I, cellx, celly, cellz, I.cell[0], I.cell[1], I.cell[2]);
+    if (!geom) {
+      // GetRandomVelocity / SampleDensityFunction (dsmc_kernel.cu:394-415)
+      float vv = sqrtf(2.0f * kBoltzRef * temp / I.mass * (-dm_logf(mt_float(rng))));
+      float vx = vv * dm_sinf(mt_float(rng) * twopi) + sux;
+      float vy = vv * dm_sinf(mt_float(rng) * twopi) + suy;
+      float vz = vv * dm_sinf(mt_float(rng) * twopi) + suz;
+      // GetRandomPosition (dsmc_kernel.cu:417-427)
+      float px = cellx + mt_float(rng) * I.cell[0];
+      float py = celly + mt_float(rng) * I.cell[1];
+      float pz = cellz + mt_float(rng) * I.cell[2];
+      I.px[par][idx] = px; I.py[par][idx] = py; I.pz[par][idx] = pz;
+      I.vx[par][idx] = vx; I.vy[par][idx] = vy; I.vz[par][idx] = vz;
+    } else {
+      I.px[par][idx] = 0.0f; I.py[par][idx] = 0.0f; I.pz[par][idx] = 0.0f;
+      I.vx[par][idx] = 0.0f; I.vy[par][idx] = 0.0f; I.vz[par][idx] = 0.0f;
+    }
+  }
+  mt_store(I, stream, rng);
+}
+
+// Move with dt = 0 after loading (include/gpu_dsmc.inl:79): box functor and
+// walls on the loaded state, in place.
+__global__ void k_move_zero(const InstDesc* __restrict__ Ip, uint32_t par, uint32_t n) {
+  const InstDesc& I = *Ip;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float x = I.px[par][i], y = I.py[par][i], z = I.pz[par][i];
+  float ux = I.vx[par][i], uy = I.vy[par][i], uz = I.vz[par][i];
+  move_reference(I, x, y, z, ux, uy, uz, 0.0f);
+  I.px[par][i] = x; I.py[par][i] = y; I.pz[par][i] = z;
+  I.vx[par][i] = ux; I.vy[par][i] = uy; I.vz[par][i] = uz;
+}
+
+// PHYSICS loading, pass 1: particles per cell by stochastic rounding of
+// n * V_fluid / Fn (per-cell density when fields are given), plus a uniform
+// random NTC remainder. Stream = cell mod n_streams, cells in increasing order.
+__global__ void k_load_count(const InstDesc* __restrict__ Ip, const float* __restrict__ fn) {
+  const InstDesc& I = *Ip;
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= I.n_streams) return;
+  Mt rng;
+  mt_load(I, s, rng);
+  for (uint32_t c = s; c < I.ncells; c += I.n_streams) {
+    float dens = fn ? fn[c] : I.n_inf;
+    float frac = I.fluid_frac ? I.fluid_frac[c] : 1.0f;
+    float lam = (float)((double)dens * (double)I.cell_volume * (double)frac / (double)I.fnum);
+    float r = mt_float(rng);
+    uint32_t k = frac > 0.0f ? dm_f2u_sat(lam + r) : 0u;
+    I.count[c] = k;
+    float2 nt = I.ntc[c];
+    nt.y = mt_float(rng);
+    I.ntc[c] = nt;
+  }
+  mt_store(I, s, rng);
+}
+
+// PHYSICS loading, pass 2: positions uniform in the cell (rejecting points in
+// the body), velocities Maxwellian (Box-Muller) about the stream velocity.
+__global__ void k_load_generate(const InstDesc* __restrict__ Ip, uint32_t par, const float* __restrict__ fu,
+                                const float* __restrict__ fv, const float* __restrict__ fw,
+                                const float* __restrict__ fT) {
+  const InstDesc& I = *Ip;
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= I.n_streams) return;
+  Mt rng;
+  mt_load(I, s, rng);
+  const float twopi = 6.28318530717958647692f;
+  const uint32_t X = I.dim[0], Y = I.dim[1];
+  for (uint32_t c = s; c < I.ncells; c += I.n_streams) {
+    uint32_t off = I.cell_start[c], k = I.cell_start[c + 1] - off;
+    if (k == 0) continue;
+    uint32_t cz = c / (X * Y), rem = c - cz * X * Y, cy = rem / X, cx = rem - cy * X;
+    float bx = I.gmin[0] + (float)cx * I.cell[0];
+    float by = I.gmin[1] + (float)cy * I.cell[1];
+    float bz = I.gmin[2] + (float)cz * I.cell[2];
+    float ux = fu ? fu[c] : I.U[0], uy = fv ? fv[c] : I.U[1], uz = fw ? fw[c] : I.U[2];
+    float T = fT ? fT[c] : I.T_inf;
+    float sd = sqrtf((float)(kBoltz * (double)T / (double)I.mass));
+    bool cut = I.fluid_frac && I.fluid_frac[c] < 1.0f;
+    for (uint32_t j = 0; j < k; ++j) {
+      float px, py, pz;
+      int tries = 0;
+      do {
+        px = bx + mt_float(rng) * I.cell[0];
+        py = by + mt_float(rng) * I.cell[1];
+        pz = bz + mt_float(rng) * I.cell[2];
+        ++tries;
+      } while (cut && tries < 64 && inside_body(I, px, py, pz));
+      float r1 = mt_float(rng), r2 = mt_float(rng), r3 = mt_float(rng), r4 = mt_float(rng);
+      float a1 = sqrtf(-2.0f * dm_logf(r1)), a2 = sqrtf(-2.0f * dm_logf(r3));
+      float s1, c1, s2, c2;
+      dm_sincosf(twopi * r2, &s1, &c1);
+      dm_sincosf(twopi * r4, &s2, &c2);
+      uint32_t idx = off + j;
+      I.px[par][idx] = px; I.py[par][idx] = py; I.pz[par][idx] = pz;
+      I.vx[par][idx] = ux + sd * a1 * c1;
+      I.vy[par][idx] = uy + sd * a1 * s1;
+      I.vz[par][idx] = uz + sd * a2 * c2;
+      I.key[par][idx] = c;
+    }
+  }
+  mt_store(I, s, rng);
+}
+
+// ================================================================= injection
+// Freestream inflow through open faces (PHYSICS): face cell g of face f
+// injects floor(lambda_f + R) particles with the flux-weighted half-range
+// Maxwellian normal velocity (acceptance-rejection on the speed-ratio
+// density z exp(-(z - s)^2), Bird 1994 sec. 12.1) and Gaussian tangential
+// components, placed on the face and advanced a random fraction of dt.
+__device__ __forceinline__ uint32_t face_of(const InstDesc& I, uint32_t g) {
+  uint32_t f = 0;
+  while (f < 5 && g >= I.face_off[f + 1]) ++f;
+  return f;
+}
+
+__global__ void k_inject_count(const InstDesc* __restrict__ Ip) {
+  const InstDesc& I = *Ip;
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= I.n_streams) return;
+  if (s >= I.n_face_cells) return;
+  Mt rng;
+  mt_load(I, s, rng);
+  for (uint32_t g = s; g < I.n_face_cells; g += I.n_streams) {
+    uint32_t f = face_of(I, g);
+    float lam = I.inj_lambda[f];
+    I.face_count[g] = dm_f2u_sat(lam + mt_float(rng));
+  }
+  mt_store(I, s, rng);
+}
+
+__global__ void k_inject_generate(const InstDesc* __restrict__ Ip, uint32_t opar) {
+  const InstDesc& I = *Ip;
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t base = I.ctr[1];
+  if (s == 0) {
+    uint32_t tot = I.face_count[I.n_face_cells];
+    uint32_t end = base + tot;
+    if (end > I.cap || end < base) {  // capacity exceeded: drop the excess, flag it
+      end = I.cap;
+      I.ctr[3] = 1u;
+    }
+    I.ctr[0] = end;
+    I.ctr[2] += end - base;
+  }
+  if (s >= I.n_streams || s >= I.n_face_cells) return;
+  Mt rng;
+  mt_load(I, s, rng);
+  const float twopi = 6.28318530717958647692f;
+  const float vm = I.vm_inf;
+  const float sd = vm * 0.70710678118654752f;  // sqrt(kT/m)
+  for (uint32_t g = s; g < I.n_face_cells; g += I.n_streams) {
+    uint32_t off = I.face_count[g], k = I.face_count[g + 1] - off;
+    if (k == 0) continue;
+    uint32_t f = face_of(I, g);
+    uint32_t local = g - I.face_off[f];
+    uint32_t axis = f >> 1;           // normal axis
+    uint32_t ta = axis == 0 ? 1u : 0u;  // first tangential axis
+    uint32_t tb = axis == 2 ? 1u : 2u;  // second tangential axis
+    uint32_t na = I.dim[ta];
+    uint32_t ia = local % na, ib = local / na;
+    float sign = (f & 1u) ? -1.0f : 1.0f;  // inward normal direction
+    float un = sign * I.U[axis];           // inward stream component
+    float sr = un / vm;                    // speed ratio of the inflow
+    float zlo = sr - 5.0f;
+    zlo = zlo > 0.0f ? zlo : 0.0f;
+    float zhi = sr + 5.0f;
+    zhi = zhi > 1.0e-3f ? zhi : 1.0e-3f;
+    float zs = 0.5f * (sr + sqrtf(sr * sr + 2.0f));  // mode of z exp(-(z-s)^2)
+    zs = zs < zlo ? zlo : (zs > zhi ? zhi : zs);
+    float gmax = zs * dm_expf(-(zs - sr) * (zs - sr));
+    for (uint32_t j = 0; j < k; ++j) {
+      float z;
+      for (int it = 0; it < 1000; ++it) {
+        z = zlo + (zhi - zlo) * mt_float(rng);
+        float gz = z * dm_expf(-(z - sr) * (z - sr));
+        if (gz > gmax * mt_float(rng)) break;
+      }
+      float vn = z * vm;
+      float r1 = mt_float(rng), r2 = mt_float(rng);
+      float a1 = sqrtf(-2.0f * dm_logf(r1));
+      float s1, c1;
+      dm_sincosf(twopi * r2, &s1, &c1);
+      float p[3], v[3];
+      v[axis] = sign * vn;
+      v[ta] = I.U[ta] + sd * a1 * c1;
+      v[tb] = I.U[tb] + sd * a1 * s1;
+      p[axis] = (f & 1u) ? I.gmax[axis] : I.gmin[axis];
+      p[ta] = I.gmin[ta] + ((float)ia + mt_float(rng)) * I.cell[ta];
+      p[tb] = I.gmin[tb] + ((float)ib + mt_float(rng)) * I.cell[tb];
+      float fr = mt_float(rng) * I.dt;
+      float x = p[0] + v[0] * fr, y = p[1] + v[1] * fr, zz = p[2] + v[2] * fr;
+      x = x < I.gmin[0] ? I.gmin[0] : (x > I.gmax[0] ? I.gmax[0] : x);
+      y = y < I.gmin[1] ? I.gmin[1] : (y > I.gmax[1] ? I.gmax[1] : y);
+      zz = zz < I.gmin[2] ? I.gmin[2] : (zz > I.gmax[2] ? I.gmax[2] : zz);
+      uint32_t idx = base + off + j;
+      if (idx >= I.cap) continue;
+      I.px[opar][idx] = x; I.py[opar][idx] = y; I.pz[opar][idx] = zz;
+      I.vx[opar][idx] = v[0]; I.vy[opar][idx] = v[1]; I.vz[opar][idx] = v[2];
+      uint32_t key = cell_key(I, x, y, zz);
+      I.key[opar][idx] = key;
+      atomicAdd(&I.count[key], 1u);
+    }
+  }
+  mt_store(I, s, rng);
+}
+
+// ================================================================== utility
+// Count particles with non-finite state or outside the box.
+__global__ void k_validate(const InstDesc* __restrict__ Ip, uint32_t par, uint32_t n, uint32_t* bad) {
+  const InstDesc& I = *Ip;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (I.key[par][i] == kInvalidKey) return;
+  float x = I.px[par][i], y = I.py[par][i], z = I.pz[par][i];
+  float a = I.vx[par][i], b = I.vy[par][i], c = I.vz[par][i];
+  bool ok = (x - x == 0.0f) && (y - y == 0.0f) && (z - z == 0.0f) && (a - a == 0.0f) && (b - b == 0.0f) &&
+            (c - c == 0.0f);
+  if (I.mode == kModePhysics)
+    ok = ok && x >= I.gmin[0] && x <= I.gmax[0] && y >= I.gmin[1] && y <= I.gmax[1] && z >= I.gmin[2] &&
+         z <= I.gmax[2];
+  if (!ok) atomicAdd(bad, 1u);
+}
+
+// One extra sample of the current state (SampleCellStatistics outside the
+// step, dsmc_kernel.cu:892-910): cells of the last ordering, current velocities.
+__global__ void k_sample_cells(const InstDesc* __restrict__ Ip, uint32_t par, uint32_t mode) {
+  const InstDesc& I = *Ip;
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= I.ncells) return;
+  uint32_t a = I.cell_start[c], b = I.cell_start[c + 1];
+  if (b <= a) return;
+  const uint32_t C = I.ncells;
+  const float* vx = I.vx[par];
+  const float* vy = I.vy[par];
+  const float* vz = I.vz[par];
+  if (mode == kModeReference) {
+    float* S = reinterpret_cast<float*>(I.samp);
+    float sn = S[c], sx = S[C + c], sy = S[2 * C + c], sz = S[3 * C + c], s2 = S[4 * C + c];
+    for (uint32_t q = a; q < b; ++q) {
+      float ux = vx[q], uy = vy[q], uz = vz[q];
+      sn += 1.0f; sx += ux; sy += uy; sz += uz;
+      s2 += ux * ux + uy * uy + uz * uz;
+    }
+    S[c] = sn; S[C + c] = sx; S[2 * C + c] = sy; S[3 * C + c] = sz; S[4 * C + c] = s2;
+  } else {
+    double* S = I.samp;
+    double sn = S[c], sx = S[C + c], sy = S[2 * C + c], sz = S[3 * C + c], s2 = S[4 * C + c];
+    for (uint32_t q = a; q < b; ++q) {
+      double ux = vx[q], uy = vy[q], uz = vz[q];
+      sn += 1.0; sx += ux; sy += uy; sz += uz;
+      s2 += ux * ux + uy * uy + uz * uz;
+    }
+    S[c] = sn; S[C + c] = sx; S[2 * C + c] = sy; S[3 * C + c] = sz; S[4 * C + c] = s2;
+  }
+}
+
+// ================================================================ launchers
+namespace dsmc_launch {
+
+static inline uint32_t blocks_for(uint32_t n, uint32_t t) { return (n + t - 1u) / t; }
+
+void mt_seed(uint32_t* state, const uint32_t* seeds, uint32_t n_streams, hipStream_t st) {
+  if (!n_streams) return;
+  hipLaunchKernelGGL(k_mt_seed, dim3(blocks_for(n_streams, 256)), dim3(256), 0, st, state, seeds, n_streams);
+}
+
+void mt_generate(const uint32_t* a, const uint32_t* b, const uint32_t* c, const uint32_t* seeds, uint32_t n_streams,
+                 uint32_t n_draws, uint32_t* out, hipStream_t st) {
+  if (!n_streams) return;
+  hipLaunchKernelGGL(k_mt_generate, dim3(blocks_for(n_streams, 256)), dim3(256), 0, st, a, b, c, seeds, n_streams,
+                     n_draws, out);
+}
+
+void cell_ids(const InstDesc& I, const float* x, const float* y, const float* z, uint32_t n, uint32_t* keys,
+              hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_cell_ids, dim3(blocks_for(n, 256)), dim3(256), 0, st, I, x, y, z, n, keys);
+}
+
+void keys_count(const InstDesc* d, uint32_t par, uint32_t n, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_keys_count, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, par, n);
+}
+
+void count_keys(const uint32_t* keys, uint32_t n, uint32_t* cnt, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_count_keys, dim3(blocks_for(n, 256)), dim3(256), 0, st, keys, n, cnt);
+}
+
+int scan(uint32_t* in, uint32_t n, uint32_t* out, uint32_t* cursor, bool clear_in, uint32_t* scratch,
+         uint32_t* total_out, hipStream_t st) {
+  uint32_t tiles = blocks_for(n > 0 ? n : 1u, kScanTile);
+  if (tiles > kScanMaxTiles) return -1;
+  uint32_t* total = total_out ? total_out : scratch + kScanMaxTiles;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(kScanThreads), 0, st, in, n, scratch);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, st, scratch, tiles, total);
+  hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanThreads), 0, st, in, n, scratch, out, cursor,
+                     clear_in ? 1u : 0u, total);
+  return 0;
+}
+
+void scatter(const InstDesc* d, uint32_t par, uint32_t grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, d, par);
+}
+
+void scatter_keys(const uint32_t* keys, uint32_t n, uint32_t* cursor, uint32_t* perm, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_scatter_keys, dim3(blocks_for(n, 256)), dim3(256), 0, st, keys, n, cursor, perm);
+}
+
+void fix_segments(const uint32_t* start, uint32_t n_cells, uint32_t* perm, hipStream_t st) {
+  if (!n_cells) return;
+  hipLaunchKernelGGL(k_fix_segments, dim3(blocks_for(n_cells, 256)), dim3(256), 0, st, start, n_cells, perm);
+}
+
+void tile(uint32_t mode, const InstDesc* d, uint32_t par, uint32_t flags, uint32_t n_vwg, hipStream_t st) {
+  if (mode == kModeReference)
+    hipLaunchKernelGGL(k_tile<kModeReference>, dim3(n_vwg), dim3(kTileThreads), 0, st, d, par, flags);
+  else
+    hipLaunchKernelGGL(k_tile<kModePhysics>, dim3(n_vwg), dim3(kTileThreads), 0, st, d, par, flags);
+}
+
+void load_reference(const InstDesc* d, uint32_t par, uint32_t ppc, uint32_t n_threads, float sux, float suy,
+                    float suz, float temp, hipStream_t st) {
+  if (!n_threads) return;
+  hipLaunchKernelGGL(k_load_reference, dim3(blocks_for(n_threads, 128)), dim3(128), 0, st, d, par, ppc, sux, suy,
+                     suz, temp);
+}
+
+void move_zero(const InstDesc* d, uint32_t par, uint32_t n, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_move_zero, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, par, n);
+}
+
+void load_count(const InstDesc* d, uint32_t n_streams, const float* fn, hipStream_t st) {
+  hipLaunchKernelGGL(k_load_count, dim3(blocks_for(n_streams, 256)), dim3(256), 0, st, d, fn);
+}
+
+void load_generate(const InstDesc* d, uint32_t par, uint32_t n_streams, const float* fu, const float* fv,
+                   const float* fw, const float* fT, hipStream_t st) {
+  hipLaunchKernelGGL(k_load_generate, dim3(blocks_for(n_streams, 256)), dim3(256), 0, st, d, par, fu, fv, fw, fT);
+}
+
+void inject_count(const InstDesc* d, uint32_t n_threads, hipStream_t st) {
+  if (!n_threads) return;
+  hipLaunchKernelGGL(k_inject_count, dim3(blocks_for(n_threads, 256)), dim3(256), 0, st, d);
+}
+
+void inject_generate(const InstDesc* d, uint32_t opar, uint32_t n_threads, hipStream_t st) {
+  if (!n_threads) return;
+  hipLaunchKernelGGL(k_inject_generate, dim3(blocks_for(n_threads, 256)), dim3(256), 0, st, d, opar);
+}
+
+void sample_cells(const InstDesc* d, uint32_t par, uint32_t mode, uint32_t ncells, hipStream_t st) {
+  if (!ncells) return;
+  hipLaunchKernelGGL(k_sample_cells, dim3(blocks_for(ncells, 256)), dim3(256), 0, st, d, par, mode);
+}
+
+void validate(const InstDesc* d, uint32_t par, uint32_t n, uint32_t* bad, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_validate, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, par, n, bad);
+}
+
+}  // namespace dsmc_launch

@@ -1,0 +1,1122 @@
+// Solver object and C ABI (include/dsmc/dsmc_c.h).
+//
+// Host orchestration of one DSMC instance on one GPU: owns the device
+// buffers, the RNG streams, the collision grid and the step sequence. It
+// replaces GPUDSMCSolver (include/gpu_dsmc.h, include/gpu_dsmc.inl) and the
+// DSMCSolver base (kernels/dsmc_base.cu); the reference's module globals
+// (dsmc_kernel.cu:117-137) become per-handle members.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dsmc/dsmc_c.h"
+#include "host/host_internal.h"
+#include "kernels/launch.h"
+
+using namespace dsmc_dev;
+
+namespace dsmc_host {
+
+static thread_local std::string g_err;
+int set_global_error(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+const char* global_error() { return g_err.c_str(); }
+
+}  // namespace dsmc_host
+
+using namespace dsmc_host;
+
+struct dsmc_solver {
+  std::string err;
+  dsmc_settings s{};
+  bool inited = false, have_state = false;
+  hipStream_t stream = nullptr;
+  InstDesc h{};
+  InstDesc* d_desc = nullptr;
+  std::vector<void*> bufs;      // lifetime of the handle
+  std::vector<void*> geom_bufs; // replaced by set_geometry
+  Mesh mesh;
+  CGrid cg;
+  bool have_geom = false;
+  std::vector<float> frac;
+  uint32_t par = 0;
+  uint64_t steps = 0;
+  uint32_t samples = 0;
+  bool sampling = false;
+  uint64_t sample_steps0 = 0;
+  double acc_snap[kAccSlots] = {0};
+  uint32_t* scan_scratch = nullptr;
+  uint32_t scatter_grid = 0;
+  uint32_t ppc = 0;
+  float ntc_init = 0.0f;
+  bool timing = false;
+  float t_ms[5] = {0, 0, 0, 0, 0};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  uint32_t debug_flags = 0;
+  std::vector<uint32_t> rng_a, rng_b, rng_c;
+
+  int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+  int hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return DSMC_OK;
+    return fail(DSMC_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+  }
+  template <class T>
+  int alloc(T** p, size_t count, bool geometry = false) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) return hip(e, "hipMalloc");
+    (geometry ? geom_bufs : bufs).push_back(q);
+    *p = (T*)q;
+    return DSMC_OK;
+  }
+  void free_geom() {
+    for (void* p : geom_bufs) (void)hipFree(p);
+    geom_bufs.clear();
+  }
+  ~dsmc_solver() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    free_geom();
+    for (void* p : bufs) (void)hipFree(p);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  int upload_desc() { return hip(hipMemcpyAsync(d_desc, &h, sizeof(h), hipMemcpyHostToDevice, stream), "desc upload"); }
+  uint32_t get_ctr(int i) {
+    uint32_t v = 0;
+    (void)hipMemcpyAsync(&v, h.ctr + i, 4, hipMemcpyDeviceToHost, stream);
+    (void)hipStreamSynchronize(stream);
+    return v;
+  }
+  bool physics() const { return s.mode == DSMC_MODE_PHYSICS; }
+};
+
+#define CHECK_RC(expr)          \
+  do {                          \
+    int rc__ = (expr);          \
+    if (rc__ != DSMC_OK) return rc__; \
+  } while (0)
+#define HIPC(expr) CHECK_RC(h->hip((expr), #expr))
+
+namespace {
+
+const double kPi = 3.14159265358979323846;
+
+// Gamma function for the VHS constant (Lanczos approximation, g = 7).
+double gamma_fn(double x) {
+  static const double c[9] = {0.99999999999980993, 676.5203681218851, -1259.1392167224028,
+                              771.32342877765313, -176.61502916214059, 12.507343278686905,
+                              -0.13857109526572012, 9.9843695780195716e-6, 1.5056327351493116e-7};
+  if (x < 0.5) return kPi / (std::sin(kPi * x) * gamma_fn(1.0 - x));
+  x -= 1.0;
+  double a = c[0];
+  double t = x + 7.5;
+  for (int i = 1; i < 9; ++i) a += c[i] / (x + i);
+  return std::sqrt(2.0 * kPi) * std::pow(t, x + 0.5) * std::exp(-t) * a;
+}
+
+// Bird's freestream number flux through a face per unit area (1994, eq. 4.22):
+// n * c_m / (2 sqrt(pi)) * [exp(-s^2) + sqrt(pi) s (1 + erf(s))], s = inward
+// speed ratio.
+double inflow_flux(double n, double vm, double un) {
+  double s = un / vm;
+  return n * vm / (2.0 * std::sqrt(kPi)) * (std::exp(-s * s) + std::sqrt(kPi) * s * (1.0 + std::erf(s)));
+}
+
+uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1u) / m * m; }
+
+int step_once(dsmc_solver* h);
+
+}  // namespace
+
+extern "C" {
+
+const char* dsmc_version(void) { return "dsmc-mi355x 0.1 (gfx950)"; }
+const char* dsmc_global_last_error(void) { return global_error(); }
+
+int dsmc_create(int solver_type, dsmc_solver** out) {
+  if (!out) return set_global_error(DSMC_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  if (solver_type == DSMC_SOLVER_CPU)
+    return set_global_error(DSMC_ERR_UNSUPPORTED, "no CPU solver (the reference's CreateSolver(SolverCPU) returns none)");
+  if (solver_type == DSMC_SOLVER_GPU_VBO)
+    return set_global_error(DSMC_ERR_UNSUPPORTED, "the OpenGL VBO solver is not part of this engine");
+  if (solver_type != DSMC_SOLVER_GPU) return set_global_error(DSMC_ERR_INVALID, "unknown solver type");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return set_global_error(DSMC_ERR_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e));
+  auto* h = new dsmc_solver();
+  e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete h;
+    return set_global_error(DSMC_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = h;
+  return DSMC_OK;
+}
+
+void dsmc_destroy(dsmc_solver* h) { delete h; }
+
+const char* dsmc_last_error(const dsmc_solver* h) { return h ? h->err.c_str() : global_error(); }
+
+int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
+  if (!h || !sp) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  if (h->inited) return h->fail(DSMC_ERR_STATE, "solver already initialised");
+  dsmc_settings s = *sp;
+  for (int a = 0; a < 3; ++a)
+    if (s.grid_dim[a] <= 0 || !(s.bmax[a] > s.bmin[a]))
+      return h->fail(DSMC_ERR_INVALID, "grid dimensions and bounds must be positive");
+  if (!(s.dt > 0.0f)) return h->fail(DSMC_ERR_INVALID, "dt must be positive");
+  if (!(s.molmass > 0.0f) || !(s.diameter > 0.0f)) return h->fail(DSMC_ERR_INVALID, "bad gas properties");
+  for (int a = 0; a < 3; ++a) s.cell_size[a] = (s.bmax[a] - s.bmin[a]) / (float)s.grid_dim[a];
+  const uint64_t C64 = (uint64_t)s.grid_dim[0] * s.grid_dim[1] * s.grid_dim[2];
+  if (C64 > 16000000ull) return h->fail(DSMC_ERR_CAPACITY, "too many cells (%llu)", (unsigned long long)C64);
+  const uint32_t C = (uint32_t)C64;
+  const bool phys = s.mode == DSMC_MODE_PHYSICS;
+  InstDesc& I = h->h;
+  std::memset(&I, 0, sizeof(I));
+  for (int a = 0; a < 3; ++a) {
+    I.gmin[a] = s.bmin[a];
+    I.gmax[a] = s.bmax[a];
+    I.cell[a] = s.cell_size[a];
+    I.dim[a] = (uint32_t)s.grid_dim[a];
+  }
+  I.ncells = C;
+  I.ntiles = (C + kTileCells - 1) / kTileCells;
+  I.cell_volume = s.cell_size[0] * s.cell_size[1] * s.cell_size[2];
+  I.mass = s.molmass;
+  I.dt = s.dt;
+  I.mode = phys ? kModePhysics : kModeReference;
+  I.periodic = (uint32_t)(unsigned char)s.periodic_condition;
+  for (int f = 0; f < 6; ++f) I.face_bc[f] = (uint32_t)s.face_bc[f];
+  I.wall_temp = s.wall_temp;
+  I.vm_wall = s.wall_temp > 0.0f ? (float)std::sqrt(2.0 * kBoltz * s.wall_temp / s.molmass) : 0.0f;
+  // InitSimulationProperties (dsmc_kernel.cu:309-348), FP32 as written there
+  I.sigma_hs = kPiRef * s.diameter * s.diameter;
+  {
+    float d = s.diameter;
+    h->ntc_init = d * d * kPiRef * sqrtf((float)(s.temp / 300.0)) * 300.0f;
+  }
+  if (phys) {
+    // VHS: sigma c_r = pi d^2 (2 k Tref / (m_r c_r^2))^(w - 1/2) / Gamma(5/2 - w) * c_r
+    double w = s.omega, mr = 0.5 * s.molmass;
+    double k = kPi * (double)s.diameter * s.diameter * std::pow(2.0 * kBoltz * s.Tref / mr, w - 0.5) /
+               gamma_fn(2.5 - w);
+    I.vhs_k = (float)k;
+    I.vhs_exp = (float)(2.0 - 2.0 * w);
+    if (w == 0.5) {
+      I.vhs_k = (float)(kPi * (double)s.diameter * s.diameter);
+      I.vhs_exp = 1.0f;
+    }
+    I.n_inf = s.density;
+    I.T_inf = s.temp;
+    for (int a = 0; a < 3; ++a) I.U[a] = s.stream_velocity[a];
+    I.vm_inf = (float)std::sqrt(2.0 * kBoltz * s.temp / s.molmass);
+    // initial (sigma c_r)max from a generous relative speed
+    double umag = std::sqrt((double)I.U[0] * I.U[0] + (double)I.U[1] * I.U[1] + (double)I.U[2] * I.U[2]);
+    double cr0 = 4.0 * I.vm_inf + umag;
+    h->ntc_init = (float)(I.vhs_exp == 1.0f ? I.vhs_k * cr0 : I.vhs_k * std::pow(cr0, (double)I.vhs_exp));
+    I.fnum = s.fnum;
+  } else {
+    if (s.part_per_cell == 0 || s.num_particles == 0) {
+      int rc = dsmc_settings_derive_particles(&s);
+      if (rc) return h->fail(rc, "%s", global_error());
+    }
+    h->ppc = s.part_per_cell;
+    I.avg_conc = (float)s.part_per_cell;
+    I.fnum = s.fnum;
+  }
+  // capacity and RNG streams
+  uint32_t cap;
+  if (phys) {
+    double vol = (double)(s.bmax[0] - s.bmin[0]) * (s.bmax[1] - s.bmin[1]) * (s.bmax[2] - s.bmin[2]);
+    double n_exp = s.target_particles ? (double)s.target_particles : (double)s.density * vol / s.fnum;
+    cap = s.capacity ? s.capacity : (uint32_t)std::min(4.0e9, 1.6 * n_exp + 65536.0);
+  } else {
+    cap = s.capacity ? std::max(s.capacity, s.num_particles) : std::max<uint32_t>(s.num_particles, 1u);
+  }
+  uint32_t ns = s.n_streams ? round_up(s.n_streams, kTileThreads) : 262144u;
+  if (!phys) ns = std::max(ns, round_up(std::max(C, (uint32_t)(s.grid_dim[0] * s.grid_dim[1]) * h->ppc), kTileThreads));
+  I.cap = cap;
+  I.n_streams = ns;
+  I.n_vwg = ns / kTileThreads;
+  h->s = s;
+
+  // ---- device buffers
+  for (int b = 0; b < 2; ++b) {
+    CHECK_RC(h->alloc(&I.px[b], cap));
+    CHECK_RC(h->alloc(&I.py[b], cap));
+    CHECK_RC(h->alloc(&I.pz[b], cap));
+    CHECK_RC(h->alloc(&I.vx[b], cap));
+    CHECK_RC(h->alloc(&I.vy[b], cap));
+    CHECK_RC(h->alloc(&I.vz[b], cap));
+    CHECK_RC(h->alloc(&I.key[b], cap));
+  }
+  CHECK_RC(h->alloc(&I.perm, cap));
+  CHECK_RC(h->alloc(&I.perm_tmp, cap));
+  CHECK_RC(h->alloc(&I.owner_tmp, cap));
+  CHECK_RC(h->alloc(&I.hits, cap));
+  CHECK_RC(h->alloc(&I.count, C));
+  CHECK_RC(h->alloc(&I.cursor, C));
+  CHECK_RC(h->alloc(&I.cell_start, (size_t)C + 1));
+  CHECK_RC(h->alloc(&I.ntc, C));
+  CHECK_RC(h->alloc(&I.cstat, C));
+  CHECK_RC(h->alloc(&I.samp, (size_t)5 * C));
+  CHECK_RC(h->alloc(&I.ctr, 16));
+  CHECK_RC(h->alloc(&I.wg_acc, (size_t)I.n_vwg * kAccSlots));
+  CHECK_RC(h->alloc(&I.mt_state, (size_t)kMtWords * ns));
+  uint32_t *ma, *mb, *mc, *seeds;
+  CHECK_RC(h->alloc(&ma, ns));
+  CHECK_RC(h->alloc(&mb, ns));
+  CHECK_RC(h->alloc(&mc, ns));
+  CHECK_RC(h->alloc(&seeds, ns));
+  I.mt_a = ma;
+  I.mt_b = mb;
+  I.mt_c = mc;
+  CHECK_RC(h->alloc(&h->scan_scratch, dsmc_launch::kScanMaxTiles + 1));
+  CHECK_RC(h->alloc(&h->d_desc, 1));
+  hipStream_t st = h->stream;
+  HIPC(hipMemsetAsync(I.count, 0, (size_t)C * 4, st));
+  HIPC(hipMemsetAsync(I.cell_start, 0, ((size_t)C + 1) * 4, st));
+  HIPC(hipMemsetAsync(I.cstat, 0, (size_t)C * sizeof(float4), st));
+  HIPC(hipMemsetAsync(I.samp, 0, (size_t)5 * C * 8, st));
+  HIPC(hipMemsetAsync(I.ctr, 0, 64, st));
+  HIPC(hipMemsetAsync(I.wg_acc, 0, (size_t)I.n_vwg * kAccSlots * 8, st));
+  HIPC(hipMemsetAsync(I.hits, 0, (size_t)cap * 4, st));
+  // ---- RNG parameters and seeds
+  std::vector<uint32_t> ra, rb, rc;
+  if (s.rng_param_file[0]) {
+    std::string e;
+    if (!load_mt_rows(s.rng_param_file, ra, rb, rc, e)) return h->fail(DSMC_ERR_IO, "%s", e.c_str());
+  } else {
+    uint32_t rows = 0;
+    const uint32_t* t = builtin_mt_table(&rows);
+    ra.assign(t, t + rows);
+    rb.assign(rows, 0x9d2c5680u);
+    rc.assign(rows, 0xefc60000u);
+  }
+  std::vector<uint32_t> ha(ns), hb(ns), hc(ns), hs(ns);
+  for (uint32_t i = 0; i < ns; ++i) {
+    ha[i] = ra[i % ra.size()];
+    hb[i] = rb[i % rb.size()];
+    hc[i] = rc[i % rc.size()];
+  }
+  if (!phys && s.seed == 0) {
+    // InitGPUTwisters seed chain (mtwister.cu:41-64): seed_i = lcg^(i+1)(999)
+    uint64_t seed = 999;
+    for (uint32_t i = 0; i < ns; ++i) {
+      seed = (seed * 279470273ull) % 4294967291ull;
+      hs[i] = (uint32_t)seed;
+    }
+  } else {
+    for (uint32_t i = 0; i < ns; ++i) hs[i] = s.seed + i + 1u;
+  }
+  h->rng_a = ha;
+  h->rng_b = hb;
+  h->rng_c = hc;
+  HIPC(hipMemcpyAsync(ma, ha.data(), (size_t)ns * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(mb, hb.data(), (size_t)ns * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(mc, hc.data(), (size_t)ns * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(seeds, hs.data(), (size_t)ns * 4, hipMemcpyHostToDevice, st));
+  dsmc_launch::mt_seed(I.mt_state, seeds, ns, st);
+  HIPC(hipGetLastError());
+  // ---- NTC state: InitMaximums (dsmc_kernel.cu:1104-1120). The reference
+  // draws the remainder from a zero MT state, which yields 1/2^32.
+  std::vector<float2> nt(C);
+  for (uint32_t c = 0; c < C; ++c) nt[c] = make_float2(h->ntc_init, phys ? 0.5f : 2.3283064365386963e-10f);
+  HIPC(hipMemcpyAsync(I.ntc, nt.data(), (size_t)C * sizeof(float2), hipMemcpyHostToDevice, st));
+  HIPC(hipStreamSynchronize(st));
+  h->scatter_grid = std::min<uint32_t>((cap + 255) / 256, 8192u);
+  CHECK_RC(h->upload_desc());
+  HIPC(hipStreamSynchronize(st));
+  h->inited = true;
+  return DSMC_OK;
+}
+
+int dsmc_init_from_config(dsmc_solver* h, const char* path) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  dsmc_settings s;
+  dsmc_settings_default(&s);
+  int rc = dsmc_settings_read_file(&s, path);
+  if (rc) return h->fail(rc, "%s", global_error());
+  if (s.mode != DSMC_MODE_PHYSICS) {
+    rc = dsmc_settings_derive_particles(&s);
+    if (rc) return h->fail(rc, "%s", global_error());
+  }
+  return dsmc_init(h, &s);
+}
+
+int dsmc_set_geometry(dsmc_solver* h, const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                      const float* normals, uint32_t gx, uint32_t gy, uint32_t gz) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->inited) return h->fail(DSMC_ERR_STATE, "set_geometry before init");
+  if (h->have_state) return h->fail(DSMC_ERR_STATE, "set_geometry after the particles were loaded");
+  if (!verts || !tris || nv == 0 || nt == 0 || gx == 0 || gy == 0 || gz == 0)
+    return h->fail(DSMC_ERR_INVALID, "empty mesh or grid");
+  for (uint64_t i = 0; i < 3ull * nt; ++i)
+    if (tris[i] >= nv) return h->fail(DSMC_ERR_INVALID, "triangle index out of range");
+  Mesh& m = h->mesh;
+  m.v.assign(verts, verts + 3ull * nv);
+  m.t.assign(tris, tris + 3ull * nt);
+  if (normals) m.n.assign(normals, normals + 3ull * nt);
+  else m.compute_normals();
+  if (h->physics()) build_cgrid_physics(m, gx, gy, gz, h->cg);
+  else build_cgrid_reference(m, gx, gy, gz, h->cg);
+  CGrid& g = h->cg;
+  h->free_geom();
+  InstDesc& I = h->h;
+  uint32_t* flag;
+  uint32_t *cs, *ce, *ct;
+  float4 *t0, *t1, *t2, *tn;
+  CHECK_RC(h->alloc(&flag, g.ncells(), true));
+  CHECK_RC(h->alloc(&cs, g.ncells(), true));
+  CHECK_RC(h->alloc(&ce, g.ncells(), true));
+  CHECK_RC(h->alloc(&ct, std::max<size_t>(g.tri.size(), 1), true));
+  CHECK_RC(h->alloc(&t0, nt, true));
+  CHECK_RC(h->alloc(&t1, nt, true));
+  CHECK_RC(h->alloc(&t2, nt, true));
+  CHECK_RC(h->alloc(&tn, nt, true));
+  std::vector<float4> h0(nt), h1(nt), h2(nt), hn(nt);
+  for (uint32_t i = 0; i < nt; ++i) {
+    const float* a = &m.v[3 * m.t[3 * i]];
+    const float* b = &m.v[3 * m.t[3 * i + 1]];
+    const float* c = &m.v[3 * m.t[3 * i + 2]];
+    h0[i] = make_float4(a[0], a[1], a[2], 0.0f);
+    h1[i] = make_float4(b[0], b[1], b[2], 0.0f);
+    h2[i] = make_float4(c[0], c[1], c[2], 0.0f);
+    hn[i] = make_float4(m.n[3 * i], m.n[3 * i + 1], m.n[3 * i + 2], 0.0f);
+  }
+  hipStream_t st = h->stream;
+  HIPC(hipMemcpyAsync(flag, g.flag.data(), g.flag.size() * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(cs, g.start.data(), g.start.size() * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(ce, g.end.data(), g.end.size() * 4, hipMemcpyHostToDevice, st));
+  if (!g.tri.empty()) HIPC(hipMemcpyAsync(ct, g.tri.data(), g.tri.size() * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(t0, h0.data(), nt * sizeof(float4), hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(t1, h1.data(), nt * sizeof(float4), hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(t2, h2.data(), nt * sizeof(float4), hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(tn, hn.data(), nt * sizeof(float4), hipMemcpyHostToDevice, st));
+  for (int a = 0; a < 3; ++a) {
+    I.cg_min[a] = g.mn[a];
+    I.cg_max[a] = g.mx[a];
+    I.cg_cell[a] = g.cell[a];
+    I.cg_dim[a] = g.dim[a];
+  }
+  I.cg_ncells = g.ncells();
+  I.cg_flag = flag;
+  I.cg_start = cs;
+  I.cg_end = ce;
+  I.cg_tri = ct;
+  I.tv0 = t0;
+  I.tv1 = t1;
+  I.tv2 = t2;
+  I.tn = tn;
+  I.has_geom = 1u;
+  h->have_geom = true;
+  if (h->physics()) {
+    fluid_fractions(m, g, I.gmin, I.cell, I.dim, h->frac);
+    float* df;
+    CHECK_RC(h->alloc(&df, I.ncells, true));
+    HIPC(hipMemcpyAsync(df, h->frac.data(), (size_t)I.ncells * 4, hipMemcpyHostToDevice, st));
+    I.fluid_frac = df;
+  }
+  CHECK_RC(h->upload_desc());
+  HIPC(hipStreamSynchronize(st));
+  return DSMC_OK;
+}
+
+static int finish_loading(dsmc_solver* h, uint32_t n, uint32_t* n_out) {
+  InstDesc& I = h->h;
+  hipStream_t st = h->stream;
+  uint32_t ctr[2] = {n, n};
+  HIPC(hipMemcpyAsync(I.ctr, ctr, 8, hipMemcpyHostToDevice, st));
+  HIPC(hipMemsetAsync(I.count, 0, (size_t)I.ncells * 4, st));
+  dsmc_launch::keys_count(h->d_desc, h->par, n, st);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(st));
+  h->have_state = true;
+  if (n_out) *n_out = n;
+  return DSMC_OK;
+}
+
+// Freestream injection set-up: face cells of the open faces and their
+// expected inflow per step.
+static int setup_injection(dsmc_solver* h) {
+  InstDesc& I = h->h;
+  uint32_t total = 0;
+  I.face_off[0] = 0;
+  for (int f = 0; f < 6; ++f) {
+    uint32_t axis = f / 2, ta = axis == 0 ? 1 : 0, tb = axis == 2 ? 1 : 2;
+    uint32_t n = (I.face_bc[f] == kFaceFreestream) ? I.dim[ta] * I.dim[tb] : 0u;
+    if (n) {
+      double sign = (f & 1) ? -1.0 : 1.0;
+      double flux = inflow_flux(I.n_inf, I.vm_inf, sign * I.U[axis]);
+      double area = (double)I.cell[ta] * I.cell[tb];
+      I.inj_lambda[f] = (float)(flux * area * I.dt / I.fnum);
+    } else {
+      I.inj_lambda[f] = 0.0f;
+    }
+    total += n;
+    I.face_off[f + 1] = total;
+  }
+  I.n_face_cells = total;
+  if (total) CHECK_RC(h->alloc(&I.face_count, (size_t)total + 1));
+  return DSMC_OK;
+}
+
+static int physics_fnum(dsmc_solver* h) {
+  InstDesc& I = h->h;
+  double vfl = 0.0;
+  for (uint32_t c = 0; c < I.ncells; ++c) vfl += h->frac.empty() ? 1.0 : h->frac[c];
+  vfl *= I.cell_volume;
+  if (h->s.target_particles) I.fnum = (float)((double)I.n_inf * vfl / h->s.target_particles);
+  if (!(I.fnum > 0.0f)) return h->fail(DSMC_ERR_INVALID, "fnum must be positive");
+  h->s.fnum = I.fnum;
+  return DSMC_OK;
+}
+
+int dsmc_init_state(dsmc_solver* h, float ux, float uy, float uz, uint32_t* n_out) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->inited) return h->fail(DSMC_ERR_STATE, "init_state before init");
+  if (h->have_state) return h->fail(DSMC_ERR_STATE, "state already loaded");
+  InstDesc& I = h->h;
+  hipStream_t st = h->stream;
+  if (!h->physics()) {
+    // GenerateInitialVelocities / GenInitialVelsUniform + move with dt = 0
+    // (include/gpu_dsmc.inl:69-92)
+    if (h->s.geometry && !h->have_geom) I.has_geom = 0u;
+    const uint32_t ppc = h->ppc;
+    const uint32_t n = ppc * I.ncells;
+    if (n > I.cap) return h->fail(DSMC_ERR_CAPACITY, "capacity %u < %u particles", I.cap, n);
+    CHECK_RC(h->upload_desc());
+    dsmc_launch::load_reference(h->d_desc, h->par, ppc, I.dim[0] * I.dim[1] * ppc, ux, uy, uz, h->s.temp, st);
+    dsmc_launch::move_zero(h->d_desc, h->par, n, st);
+    HIPC(hipGetLastError());
+    return finish_loading(h, n, n_out);
+  }
+  I.U[0] = ux;
+  I.U[1] = uy;
+  I.U[2] = uz;
+  CHECK_RC(physics_fnum(h));
+  CHECK_RC(setup_injection(h));
+  CHECK_RC(h->upload_desc());
+  dsmc_launch::load_count(h->d_desc, I.n_streams, nullptr, st);
+  if (dsmc_launch::scan(I.count, I.ncells, I.cell_start, nullptr, true, h->scan_scratch, I.ctr + 1, st))
+    return h->fail(DSMC_ERR_CAPACITY, "too many cells to scan");
+  uint32_t n = h->get_ctr(1);
+  if (n > I.cap) return h->fail(DSMC_ERR_CAPACITY, "capacity %u < %u particles", I.cap, n);
+  dsmc_launch::load_generate(h->d_desc, h->par, I.n_streams, nullptr, nullptr, nullptr, nullptr, st);
+  HIPC(hipGetLastError());
+  return finish_loading(h, n, n_out);
+}
+
+int dsmc_init_state_from_fields(dsmc_solver* h, const float* fn, const float* fu, const float* fv, const float* fw,
+                                const float* fT, uint32_t n_cells, uint32_t* n_out) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->inited) return h->fail(DSMC_ERR_STATE, "init_state before init");
+  if (h->have_state) return h->fail(DSMC_ERR_STATE, "state already loaded");
+  if (!h->physics()) return h->fail(DSMC_ERR_UNSUPPORTED, "field loading needs PHYSICS mode");
+  InstDesc& I = h->h;
+  if (n_cells != I.ncells || !fn || !fu || !fv || !fw || !fT) return h->fail(DSMC_ERR_INVALID, "field size mismatch");
+  for (uint32_t c = 0; c < n_cells; ++c)
+    if (!(fn[c] >= 0.0f) || !(fT[c] > 0.0f) || !std::isfinite(fu[c]) || !std::isfinite(fv[c]) || !std::isfinite(fw[c]))
+      return h->fail(DSMC_ERR_INVALID, "non-physical field value in cell %u", c);
+  hipStream_t st = h->stream;
+  CHECK_RC(physics_fnum(h));
+  CHECK_RC(setup_injection(h));
+  CHECK_RC(h->upload_desc());
+  float* d[5];
+  const float* src[5] = {fn, fu, fv, fw, fT};
+  for (int k = 0; k < 5; ++k) {
+    CHECK_RC(h->alloc(&d[k], n_cells));
+    HIPC(hipMemcpyAsync(d[k], src[k], (size_t)n_cells * 4, hipMemcpyHostToDevice, st));
+  }
+  dsmc_launch::load_count(h->d_desc, I.n_streams, d[0], st);
+  if (dsmc_launch::scan(I.count, I.ncells, I.cell_start, nullptr, true, h->scan_scratch, I.ctr + 1, st))
+    return h->fail(DSMC_ERR_CAPACITY, "too many cells to scan");
+  uint32_t n = h->get_ctr(1);
+  if (n > I.cap) return h->fail(DSMC_ERR_CAPACITY, "capacity %u < %u particles", I.cap, n);
+  dsmc_launch::load_generate(h->d_desc, h->par, I.n_streams, d[1], d[2], d[3], d[4], st);
+  HIPC(hipGetLastError());
+  return finish_loading(h, n, n_out);
+}
+
+int dsmc_step(dsmc_solver* h, float dt) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->have_state) return h->fail(DSMC_ERR_STATE, "step before init_state");
+  if (dt != h->h.dt) {
+    if (!(dt > 0.0f)) return h->fail(DSMC_ERR_INVALID, "dt must be positive");
+    h->h.dt = dt;
+    if (h->physics() && h->h.n_face_cells) {
+      // inflow per step scales with dt
+      for (int f = 0; f < 6; ++f) h->h.inj_lambda[f] = 0.0f;
+      h->h.n_face_cells = 0;
+      CHECK_RC(setup_injection(h));
+    }
+    CHECK_RC(h->upload_desc());
+  }
+  return step_once(h);
+}
+
+int dsmc_run(dsmc_solver* h, uint32_t steps, float dt) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->have_state) return h->fail(DSMC_ERR_STATE, "run before init_state");
+  if (h->timing)
+    for (auto& t : h->t_ms) t = 0.0f;
+  for (uint32_t i = 0; i < steps; ++i) CHECK_RC(dsmc_step(h, dt));
+  return DSMC_OK;
+}
+
+int dsmc_set_sampling(dsmc_solver* h, int enabled, int reset) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->inited) return h->fail(DSMC_ERR_STATE, "not initialised");
+  h->sampling = enabled != 0;
+  if (reset) {
+    InstDesc& I = h->h;
+    HIPC(hipMemsetAsync(I.samp, 0, (size_t)5 * I.ncells * 8, h->stream));
+    h->samples = 0;
+    h->sample_steps0 = h->steps;
+    std::vector<double> acc((size_t)I.n_vwg * kAccSlots);
+    HIPC(hipMemcpyAsync(acc.data(), I.wg_acc, acc.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < kAccSlots; ++k) {
+      double t = 0.0;
+      for (uint32_t w = 0; w < I.n_vwg; ++w) t += acc[(size_t)w * kAccSlots + k];
+      h->acc_snap[k] = t;
+    }
+  }
+  return DSMC_OK;
+}
+
+int dsmc_sample(dsmc_solver* h) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->have_state) return h->fail(DSMC_ERR_STATE, "sample before init_state");
+  dsmc_launch::sample_cells(h->d_desc, h->par, h->h.mode, h->h.ncells, h->stream);
+  HIPC(hipGetLastError());
+  h->samples += 1;
+  return DSMC_OK;
+}
+
+int dsmc_get_cell_stats(dsmc_solver* h, dsmc_cell_stats* out, uint32_t n) {
+  if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  if (!h->inited) return h->fail(DSMC_ERR_STATE, "not initialised");
+  InstDesc& I = h->h;
+  if (n < I.ncells) return h->fail(DSMC_ERR_INVALID, "need %u cells", I.ncells);
+  const uint32_t C = I.ncells;
+  std::vector<double> buf((size_t)5 * C);
+  HIPC(hipMemcpyAsync(buf.data(), I.samp, buf.size() * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  const float* f = reinterpret_cast<const float*>(buf.data());
+  for (uint32_t c = 0; c < C; ++c) {
+    if (h->physics()) {
+      out[c].num_in_sample = buf[c];
+      out[c].xyz_sum[0] = buf[C + c];
+      out[c].xyz_sum[1] = buf[2 * (size_t)C + c];
+      out[c].xyz_sum[2] = buf[3 * (size_t)C + c];
+      out[c].vel_mag = buf[4 * (size_t)C + c];
+    } else {
+      out[c].num_in_sample = f[c];
+      out[c].xyz_sum[0] = f[C + c];
+      out[c].xyz_sum[1] = f[2 * (size_t)C + c];
+      out[c].xyz_sum[2] = f[3 * (size_t)C + c];
+      out[c].vel_mag = f[4 * (size_t)C + c];
+    }
+  }
+  return DSMC_OK;
+}
+
+int dsmc_get_fields(dsmc_solver* h, float* n, float* u, float* v, float* w, float* T, uint32_t nc) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  InstDesc& I = h->h;
+  if (nc < I.ncells) return h->fail(DSMC_ERR_INVALID, "need %u cells", I.ncells);
+  std::vector<dsmc_cell_stats> cs(I.ncells);
+  CHECK_RC(dsmc_get_cell_stats(h, cs.data(), I.ncells));
+  const double S = h->samples > 0 ? (double)h->samples : 1.0;
+  const double kb = h->physics() ? kBoltz : (double)kBoltzRef;
+  for (uint32_t c = 0; c < I.ncells; ++c) {
+    double N = cs[c].num_in_sample;
+    double fr = (h->physics() && !h->frac.empty()) ? h->frac[c] : 1.0;
+    double vol = (double)I.cell_volume * fr;
+    double dens = vol > 0.0 ? (double)I.fnum * N / (vol * S) : 0.0;
+    double ux = 0, uy = 0, uz = 0, tt = 0;
+    if (N > 0) {
+      ux = cs[c].xyz_sum[0] / N;
+      uy = cs[c].xyz_sum[1] / N;
+      uz = cs[c].xyz_sum[2] / N;
+      tt = (double)I.mass * (cs[c].vel_mag / N - (ux * ux + uy * uy + uz * uz)) / (3.0 * kb);
+    }
+    if (n) n[c] = (float)dens;
+    if (u) u[c] = (float)ux;
+    if (v) v[c] = (float)uy;
+    if (w) w[c] = (float)uz;
+    if (T) T[c] = (float)tt;
+  }
+  return DSMC_OK;
+}
+
+static int sum_acc(dsmc_solver* h, double out[kAccSlots]) {
+  InstDesc& I = h->h;
+  std::vector<double> acc((size_t)I.n_vwg * kAccSlots);
+  HIPC(hipMemcpyAsync(acc.data(), I.wg_acc, acc.size() * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  for (int k = 0; k < kAccSlots; ++k) {
+    double t = 0.0;
+    for (uint32_t w = 0; w < I.n_vwg; ++w) t += acc[(size_t)w * kAccSlots + k];
+    out[k] = t;
+  }
+  return DSMC_OK;
+}
+
+int dsmc_get_collision_stats(dsmc_solver* h, double out[4]) {
+  if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  if (!h->inited) return h->fail(DSMC_ERR_STATE, "not initialised");
+  InstDesc& I = h->h;
+  if (!h->physics()) {
+    // GetStatistics (dsmc_kernel.cu:840-854): sum of the per-cell counters
+    std::vector<float4> cs(I.ncells);
+    HIPC(hipMemcpyAsync(cs.data(), I.cstat, cs.size() * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    double a = 0, b = 0, c = 0;
+    for (auto& x : cs) {
+      a += x.x;
+      b += x.y;
+      c += x.z;
+    }
+    out[0] = a;
+    out[1] = b;
+    out[2] = c;
+    out[3] = 0.0;
+    return DSMC_OK;
+  }
+  double acc[kAccSlots];
+  CHECK_RC(sum_acc(h, acc));
+  out[0] = acc[kAccSel];
+  out[1] = acc[kAccColl];
+  out[2] = acc[kAccCr];
+  out[3] = 0.0;
+  return DSMC_OK;
+}
+
+int dsmc_get_drag(dsmc_solver* h, double out[4]) {
+  if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  if (!h->inited) return h->fail(DSMC_ERR_STATE, "not initialised");
+  for (int k = 0; k < 4; ++k) out[k] = 0.0;
+  if (!h->physics()) return DSMC_OK;
+  double acc[kAccSlots];
+  CHECK_RC(sum_acc(h, acc));
+  double tw = (double)(h->steps - h->sample_steps0) * h->h.dt;
+  if (tw <= 0.0) return DSMC_OK;
+  double f = (double)h->h.fnum / tw;
+  out[0] = (acc[kAccFx] - h->acc_snap[kAccFx]) * f;
+  out[1] = (acc[kAccFy] - h->acc_snap[kAccFy]) * f;
+  out[2] = (acc[kAccFz] - h->acc_snap[kAccFz]) * f;
+  out[3] = (acc[kAccHeat] - h->acc_snap[kAccHeat]) * f;
+  return DSMC_OK;
+}
+
+int dsmc_dump_statistics(dsmc_solver* h, const char* path) {
+  if (!h || !path) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  if (!h->have_state) return h->fail(DSMC_ERR_STATE, "no state");
+  // DumpStatistics -> SampleStatistics samples once more without counting it
+  // (include/gpu_dsmc.inl:277-282, kernels/dsmc_base.cu:165-174)
+  double coll[4];
+  CHECK_RC(dsmc_get_collision_stats(h, coll));
+  dsmc_launch::sample_cells(h->d_desc, h->par, h->h.mode, h->h.ncells, h->stream);
+  HIPC(hipGetLastError());
+  InstDesc& I = h->h;
+  std::vector<dsmc_cell_stats> cs(I.ncells);
+  CHECK_RC(dsmc_get_cell_stats(h, cs.data(), I.ncells));
+  std::ofstream out(path);
+  if (!out) return h->fail(DSMC_ERR_IO, "cannot write %s", path);
+  // ProcessAndOutputStatistics (kernels/dsmc_base.cu:104-163)
+  const float sel = (float)coll[0], col = (float)coll[1], cr = (float)coll[2];
+  out << "Selections: " << sel << "\nCollisions: " << col << "\nMean separation: " << cr / col
+      << "\nCollision ratio: " << col / sel << "\n";
+  const float sim_time = (float)((double)h->steps * I.dt);
+  out << "Simulation time: " << sim_time << " Running time: " << 0 << "\n";
+  out << "Cell N  ||  SampNum  ||  Dens  ||          Velocity          ||  Temperature\n";
+  double tot = 0, denn = 0, avg_denn = 0;
+  const float cell_volume = I.cell_volume;
+  const float kb = kBoltzRef;
+  for (uint32_t i = 0; i < I.ncells; ++i) {
+    const dsmc_cell_stats& s = cs[i];
+    float a = h->s.fnum / (cell_volume * (float)h->samples);
+    denn = a * (float)s.num_in_sample;
+    avg_denn += denn;
+    float n = (float)s.num_in_sample;
+    float vx = (float)s.xyz_sum[0] / n, vy = (float)s.xyz_sum[1] / n, vz = (float)s.xyz_sum[2] / n;
+    float uu = vx * vx + vy * vy + vz * vz;
+    float tt = h->s.molmass * ((float)s.vel_mag / n - uu) / (3.0f * kb);
+    tot += tt;
+    out << i << "  ||  " << (uint32_t)n << "  ||  " << denn << "  ||  " << vx << "," << vy << "," << vz << "  ||  "
+        << tt << "\n";
+  }
+  tot /= I.ncells;
+  avg_denn /= I.ncells;
+  const float gv = (h->s.bmax[0] - h->s.bmin[0]) * (h->s.bmax[1] - h->s.bmin[1]) * (h->s.bmax[2] - h->s.bmin[2]);
+  const double dens2 = (double)h->s.density * h->s.diameter, diam2 = (double)h->s.diameter * h->s.density;
+  const double molmas = h->s.molmass;
+  const float Tref = h->s.Tref;
+  float tColl = (float)(2.0f * sim_time * dens2 * gv * kPiRef * diam2 * std::pow(tot / Tref, 0.25) *
+                        sqrtf((float)(kb * Tref / (kPiRef * molmas))) / h->s.fnum);
+  out << "\n\nExpected collisions: " << tColl << " Ratio: " << tColl / col << "\n";
+  out << "Average temperature: " << tot << " Reference temperature is: " << h->s.temp
+      << "\nAverage density: " << denn << " Reference density: " << h->s.density << "\n\n" << std::endl;
+  return DSMC_OK;
+}
+
+int dsmc_get_counts(dsmc_solver* h, uint64_t* steps, uint32_t* n_particles, uint32_t* samples) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (steps) *steps = h->steps;
+  if (samples) *samples = h->samples;
+  if (n_particles) {
+    *n_particles = 0;
+    if (h->have_state) {
+      // live particles = stored minus removed; the next scan's total equals it
+      InstDesc& I = h->h;
+      uint32_t stored = h->get_ctr(0);
+      std::vector<uint32_t> keys(stored);
+      if (stored)
+        HIPC(hipMemcpyAsync(keys.data(), I.key[h->par], (size_t)stored * 4, hipMemcpyDeviceToHost, h->stream));
+      HIPC(hipStreamSynchronize(h->stream));
+      uint32_t live = 0;
+      for (uint32_t k : keys) live += k != kInvalidKey;
+      *n_particles = live;
+    }
+  }
+  return DSMC_OK;
+}
+
+int dsmc_synchronize(dsmc_solver* h) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  HIPC(hipStreamSynchronize(h->stream));
+  return DSMC_OK;
+}
+
+int dsmc_validate(dsmc_solver* h, uint32_t* n_bad) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->have_state) return h->fail(DSMC_ERR_STATE, "no state");
+  uint32_t* bad = h->h.ctr + 8;
+  HIPC(hipMemsetAsync(bad, 0, 4, h->stream));
+  uint32_t n = h->get_ctr(0);
+  dsmc_launch::validate(h->d_desc, h->par, n, bad, h->stream);
+  HIPC(hipGetLastError());
+  uint32_t b = h->get_ctr(8);
+  if (n_bad) *n_bad = b;
+  return b ? h->fail(DSMC_ERR_NUMERIC, "%u particles with non-finite state or outside the box", b) : DSMC_OK;
+}
+
+int dsmc_get_particles(dsmc_solver* h, float* pos, float* vel, uint32_t* keys, uint32_t cap, uint32_t* n_out) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->have_state) return h->fail(DSMC_ERR_STATE, "no state");
+  InstDesc& I = h->h;
+  uint32_t n = h->get_ctr(0);
+  if (n_out) *n_out = n;
+  if (cap < n) return pos || vel || keys ? h->fail(DSMC_ERR_CAPACITY, "need room for %u particles", n) : DSMC_OK;
+  hipStream_t st = h->stream;
+  uint32_t p = h->par;
+  if (pos) {
+    HIPC(hipMemcpyAsync(pos, I.px[p], (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(pos + cap, I.py[p], (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(pos + 2 * (size_t)cap, I.pz[p], (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  }
+  if (vel) {
+    HIPC(hipMemcpyAsync(vel, I.vx[p], (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(vel + cap, I.vy[p], (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(vel + 2 * (size_t)cap, I.vz[p], (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  }
+  if (keys) HIPC(hipMemcpyAsync(keys, I.key[p], (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  return DSMC_OK;
+}
+
+int dsmc_set_particles(dsmc_solver* h, const float* pos, const float* vel, uint32_t n) {
+  if (!h || !pos || !vel) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  if (!h->inited) return h->fail(DSMC_ERR_STATE, "not initialised");
+  InstDesc& I = h->h;
+  if (n > I.cap) return h->fail(DSMC_ERR_CAPACITY, "capacity %u < %u", I.cap, n);
+  if (!I.has_geom && h->physics() == false && h->s.geometry && h->have_geom == false) I.has_geom = 0u;
+  if (h->physics() && I.n_face_cells == 0 && !h->have_state) {
+    CHECK_RC(physics_fnum(h));
+    CHECK_RC(setup_injection(h));
+  }
+  CHECK_RC(h->upload_desc());
+  hipStream_t st = h->stream;
+  uint32_t p = h->par;
+  HIPC(hipMemcpyAsync(I.px[p], pos, (size_t)n * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(I.py[p], pos + n, (size_t)n * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(I.pz[p], pos + 2 * (size_t)n, (size_t)n * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(I.vx[p], vel, (size_t)n * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(I.vy[p], vel + n, (size_t)n * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(I.vz[p], vel + 2 * (size_t)n, (size_t)n * 4, hipMemcpyHostToDevice, st));
+  h->have_state = false;
+  return finish_loading(h, n, nullptr);
+}
+
+int dsmc_get_cell_start(dsmc_solver* h, uint32_t* out, uint32_t n) {
+  if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  InstDesc& I = h->h;
+  if (n < I.ncells + 1) return h->fail(DSMC_ERR_INVALID, "need %u entries", I.ncells + 1);
+  HIPC(hipMemcpyAsync(out, I.cell_start, ((size_t)I.ncells + 1) * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  return DSMC_OK;
+}
+
+int dsmc_get_perm(dsmc_solver* h, uint32_t* out, uint32_t n) {
+  if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  InstDesc& I = h->h;
+  uint32_t m = std::min(n, I.cap);
+  HIPC(hipMemcpyAsync(out, I.perm, (size_t)m * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  return DSMC_OK;
+}
+
+int dsmc_get_hit_counts(dsmc_solver* h, uint32_t* out, uint32_t n) {
+  if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  InstDesc& I = h->h;
+  uint32_t m = std::min(n, I.cap);
+  HIPC(hipMemcpyAsync(out, I.hits, (size_t)m * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  return DSMC_OK;
+}
+
+int dsmc_get_ntc_state(dsmc_solver* h, float* out, uint32_t n) {
+  if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  InstDesc& I = h->h;
+  if (n < I.ncells) return h->fail(DSMC_ERR_INVALID, "need %u cells", I.ncells);
+  HIPC(hipMemcpyAsync(out, I.ntc, (size_t)I.ncells * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  return DSMC_OK;
+}
+
+int dsmc_get_rng_state(dsmc_solver* h, uint32_t* out, uint32_t n) {
+  if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  InstDesc& I = h->h;
+  size_t need = (size_t)kMtWords * I.n_streams;
+  if (n < need) return h->fail(DSMC_ERR_INVALID, "need %zu words", need);
+  std::vector<uint32_t> tmp(need);
+  HIPC(hipMemcpyAsync(tmp.data(), I.mt_state, need * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  for (uint32_t s = 0; s < I.n_streams; ++s)
+    for (uint32_t w = 0; w < kMtWords; ++w) out[(size_t)s * kMtWords + w] = tmp[(size_t)w * I.n_streams + s];
+  return DSMC_OK;
+}
+
+int dsmc_set_rng_state(dsmc_solver* h, const uint32_t* in, uint32_t n) {
+  if (!h || !in) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  InstDesc& I = h->h;
+  size_t need = (size_t)kMtWords * I.n_streams;
+  if (n < need) return h->fail(DSMC_ERR_INVALID, "need %zu words", need);
+  std::vector<uint32_t> tmp(need);
+  for (uint32_t s = 0; s < I.n_streams; ++s)
+    for (uint32_t w = 0; w < kMtWords; ++w) tmp[(size_t)w * I.n_streams + s] = in[(size_t)s * kMtWords + w];
+  HIPC(hipMemcpyAsync(I.mt_state, tmp.data(), need * 4, hipMemcpyHostToDevice, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  return DSMC_OK;
+}
+
+int dsmc_get_fluid_fraction(dsmc_solver* h, float* out, uint32_t n) {
+  if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  InstDesc& I = h->h;
+  if (n < I.ncells) return h->fail(DSMC_ERR_INVALID, "need %u cells", I.ncells);
+  for (uint32_t c = 0; c < I.ncells; ++c) out[c] = h->frac.empty() ? 1.0f : h->frac[c];
+  return DSMC_OK;
+}
+
+int dsmc_get_collision_grid(dsmc_solver* h, uint32_t* flags, uint32_t n) {
+  if (!h || !flags) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  if (!h->have_geom) return h->fail(DSMC_ERR_STATE, "no geometry");
+  if (n < h->cg.ncells()) return h->fail(DSMC_ERR_INVALID, "need %u cells", h->cg.ncells());
+  std::copy(h->cg.flag.begin(), h->cg.flag.end(), flags);
+  return DSMC_OK;
+}
+
+int dsmc_get_timings(dsmc_solver* h, float* out_ms, uint32_t n) {
+  if (!h || !out_ms) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  for (uint32_t k = 0; k < n && k < 5; ++k) out_ms[k] = h->t_ms[k];
+  return DSMC_OK;
+}
+
+int dsmc_set_timing(dsmc_solver* h, int enabled) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  h->timing = enabled != 0;
+  if (h->timing && !h->ev[0])
+    for (auto& e : h->ev) HIPC(hipEventCreate(&e));
+  for (auto& t : h->t_ms) t = 0.0f;
+  return DSMC_OK;
+}
+
+int dsmc_set_debug(dsmc_solver* h, uint32_t flags) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  h->debug_flags = flags;
+  return DSMC_OK;
+}
+
+int dsmc_get_layout(dsmc_solver* h, uint32_t* tile_cells, uint32_t* threads, uint32_t* n_streams, uint32_t* n_vwg) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (tile_cells) *tile_cells = kTileCells;
+  if (threads) *threads = kTileThreads;
+  if (n_streams) *n_streams = h->h.n_streams;
+  if (n_vwg) *n_vwg = h->h.n_vwg;
+  return DSMC_OK;
+}
+
+int dsmc_get_rng_params(dsmc_solver* h, uint32_t* a, uint32_t* b, uint32_t* c, uint32_t n) {
+  if (!h || !a || !b || !c) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  if (!h->inited) return h->fail(DSMC_ERR_STATE, "not initialised");
+  if (n < h->rng_a.size()) return h->fail(DSMC_ERR_INVALID, "need %zu streams", h->rng_a.size());
+  std::copy(h->rng_a.begin(), h->rng_a.end(), a);
+  std::copy(h->rng_b.begin(), h->rng_b.end(), b);
+  std::copy(h->rng_c.begin(), h->rng_c.end(), c);
+  return DSMC_OK;
+}
+
+int dsmc_get_derived(dsmc_solver* h, float* out, uint32_t n) {
+  if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  InstDesc& I = h->h;
+  float v[12] = {I.fnum, I.vhs_k, I.vhs_exp, h->ntc_init, I.cell_volume, I.sigma_hs,
+                 I.inj_lambda[0], I.inj_lambda[1], I.inj_lambda[2], I.inj_lambda[3], I.inj_lambda[4], I.inj_lambda[5]};
+  for (uint32_t k = 0; k < n && k < 12; ++k) out[k] = v[k];
+  return DSMC_OK;
+}
+
+int dsmc_get_collision_grid_full(dsmc_solver* h, float bounds[9], uint32_t dims[3], uint32_t* flags, uint32_t* start,
+                                 uint32_t* end, uint32_t* entries, uint32_t cap_cells, uint32_t cap_entries,
+                                 uint32_t* n_entries) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->have_geom) return h->fail(DSMC_ERR_STATE, "no geometry");
+  const CGrid& g = h->cg;
+  if (bounds)
+    for (int k = 0; k < 3; ++k) {
+      bounds[k] = g.mn[k];
+      bounds[3 + k] = g.mx[k];
+      bounds[6 + k] = g.cell[k];
+    }
+  if (dims)
+    for (int k = 0; k < 3; ++k) dims[k] = g.dim[k];
+  if (n_entries) *n_entries = (uint32_t)g.tri.size();
+  if ((flags || start || end) && cap_cells < g.ncells()) return h->fail(DSMC_ERR_CAPACITY, "cell buffers too small");
+  if (entries && cap_entries < g.tri.size()) return h->fail(DSMC_ERR_CAPACITY, "entry buffer too small");
+  if (flags) std::copy(g.flag.begin(), g.flag.end(), flags);
+  if (start) std::copy(g.start.begin(), g.start.end(), start);
+  if (end) std::copy(g.end.begin(), g.end.end(), end);
+  if (entries) std::copy(g.tri.begin(), g.tri.end(), entries);
+  return DSMC_OK;
+}
+
+// ------------------------------------------------------------- operators
+int dsmc_op_mt_generate(const uint32_t* a, const uint32_t* b, const uint32_t* c, const uint32_t* seeds,
+                        uint32_t n_streams, uint32_t n_draws, uint32_t* out, void* stream) {
+  if (!a || !b || !c || !seeds || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  dsmc_launch::mt_generate(a, b, c, seeds, n_streams, n_draws, out, (hipStream_t)stream);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_global_error(DSMC_ERR_DEVICE, hipGetErrorString(e));
+  return DSMC_OK;
+}
+
+int dsmc_op_cell_ids(const dsmc_settings* s, const float* x, const float* y, const float* z, uint32_t n,
+                     uint32_t* keys, void* stream) {
+  if (!s || !x || !y || !z || !keys) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  InstDesc I;
+  std::memset(&I, 0, sizeof(I));
+  for (int a = 0; a < 3; ++a) {
+    if (s->grid_dim[a] <= 0) return set_global_error(DSMC_ERR_INVALID, "bad grid");
+    I.gmin[a] = s->bmin[a];
+    I.gmax[a] = s->bmax[a];
+    I.cell[a] = (s->bmax[a] - s->bmin[a]) / (float)s->grid_dim[a];
+    I.dim[a] = (uint32_t)s->grid_dim[a];
+  }
+  dsmc_launch::cell_ids(I, x, y, z, n, keys, (hipStream_t)stream);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_global_error(DSMC_ERR_DEVICE, hipGetErrorString(e));
+  return DSMC_OK;
+}
+
+int dsmc_op_sort(const uint32_t* keys, uint32_t n, uint32_t n_cells, uint32_t* perm, uint32_t* cell_start,
+                 uint32_t* work, void* stream) {
+  if (!keys || !perm || !cell_start || !work) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  uint32_t* cnt = work;
+  uint32_t* cur = work + n_cells;
+  uint32_t* scr = work + 2 * (size_t)n_cells;
+  hipError_t e = hipMemsetAsync(cnt, 0, (size_t)n_cells * 4, st);
+  if (e != hipSuccess) return set_global_error(DSMC_ERR_DEVICE, hipGetErrorString(e));
+  dsmc_launch::count_keys(keys, n, cnt, st);
+  if (dsmc_launch::scan(cnt, n_cells, cell_start, cur, true, scr, nullptr, st))
+    return set_global_error(DSMC_ERR_CAPACITY, "too many cells");
+  dsmc_launch::scatter_keys(keys, n, cur, perm, st);
+  dsmc_launch::fix_segments(cell_start, n_cells, perm, st);
+  e = hipGetLastError();
+  if (e != hipSuccess) return set_global_error(DSMC_ERR_DEVICE, hipGetErrorString(e));
+  return DSMC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int step_once(dsmc_solver* h) {
+  InstDesc& I = h->h;
+  hipStream_t st = h->stream;
+  const uint64_t stepno = h->steps + 1;
+  uint32_t flags = h->debug_flags;
+  bool sample = false;
+  if (h->physics()) sample = h->sampling;
+  else {
+    uint32_t every = h->s.sample_every ? h->s.sample_every : 100u;
+    sample = (stepno % every) == 0;
+  }
+  if (sample) flags |= dsmc_launch::kFlagSample;
+  if (h->timing) HIPC(hipEventRecord(h->ev[0], st));
+  if (dsmc_launch::scan(I.count, I.ncells, I.cell_start, I.cursor, true, h->scan_scratch, I.ctr + 1, st))
+    return h->fail(DSMC_ERR_CAPACITY, "too many cells to scan");
+  if (h->timing) HIPC(hipEventRecord(h->ev[1], st));
+  dsmc_launch::scatter(h->d_desc, h->par, h->scatter_grid, st);
+  if (h->timing) HIPC(hipEventRecord(h->ev[2], st));
+  dsmc_launch::tile(I.mode, h->d_desc, h->par, flags, I.n_vwg, st);
+  if (h->timing) HIPC(hipEventRecord(h->ev[3], st));
+  if (I.n_face_cells) {
+    dsmc_launch::inject_count(h->d_desc, std::min(I.n_streams, I.n_face_cells), st);
+    if (dsmc_launch::scan(I.face_count, I.n_face_cells, I.face_count, nullptr, false, h->scan_scratch, nullptr, st))
+      return h->fail(DSMC_ERR_CAPACITY, "too many face cells");
+    dsmc_launch::inject_generate(h->d_desc, h->par ^ 1u, std::min(I.n_streams, I.n_face_cells), st);
+  }
+  if (h->timing) {
+    HIPC(hipEventRecord(h->ev[4], st));
+    HIPC(hipEventSynchronize(h->ev[4]));
+    float t;
+    for (int k = 0; k < 4; ++k) {
+      HIPC(hipEventElapsedTime(&t, h->ev[k], h->ev[k + 1]));
+      h->t_ms[k] += t;
+    }
+    HIPC(hipEventElapsedTime(&t, h->ev[0], h->ev[4]));
+    h->t_ms[4] += t;
+  }
+  HIPC(hipGetLastError());
+  h->par ^= 1u;
+  h->steps = stepno;
+  if (sample) h->samples += 1;
+  return DSMC_OK;
+}
+
+}  // namespace
