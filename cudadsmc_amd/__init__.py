@@ -292,6 +292,9 @@ class Solver:
     def set_debug(self, flags: int) -> None:
         self._c(lib().dsmc_set_debug(self._h, int(flags)))
 
+    def set_graphs(self, enabled: bool) -> None:
+        self._c(lib().dsmc_set_graphs(self._h, int(enabled)))
+
     def set_timing(self, enabled: bool) -> None:
         self._c(lib().dsmc_set_timing(self._h, int(enabled)))
 
@@ -306,6 +309,44 @@ class Solver:
         if rc not in (0, -7):
             self._c(rc)
         return n.value
+
+
+class Batch:
+    """Independent instances advanced by shared kernel launches (the ensemble step)."""
+
+    def __init__(self, solvers: list[Solver]):
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(solvers))(*[s._h.value for s in solvers])
+        self._b = C.c_void_p()
+        check(lib().dsmc_batch_create(arr, len(solvers), C.byref(self._b)))
+
+    def _c(self, rc: int) -> None:
+        if rc:
+            msg = lib().dsmc_batch_last_error(self._b) or b""
+            raise DsmcError(rc, msg.decode(errors="replace"))
+
+    def run(self, steps: int) -> None:
+        self._c(lib().dsmc_batch_run(self._b, int(steps)))
+
+    def set_sampling(self, enabled: bool, reset: bool = True) -> None:
+        self._c(lib().dsmc_batch_set_sampling(self._b, int(enabled), int(reset)))
+
+    def set_graphs(self, enabled: bool) -> None:
+        self._c(lib().dsmc_batch_set_graphs(self._b, int(enabled)))
+
+    def synchronize(self) -> None:
+        self._c(lib().dsmc_batch_synchronize(self._b))
+
+    def close(self) -> None:
+        if self._b:
+            lib().dsmc_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def CreateSolver(solver_type: int = SOLVER_GPU) -> Solver:

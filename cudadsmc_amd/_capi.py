@@ -104,6 +104,7 @@ def lib() -> C.CDLL:
         "dsmc_get_timings": ([P, pF, U32], I32),
         "dsmc_set_timing": ([P, I32], I32),
         "dsmc_set_debug": ([P, U32], I32),
+        "dsmc_set_graphs": ([P, I32], I32),
         "dsmc_get_layout": ([P, pU, pU, pU, pU], I32),
         "dsmc_get_derived": ([P, pF, U32], I32),
         "dsmc_get_rng_params": ([P, pU, pU, pU, U32], I32),
@@ -113,7 +114,14 @@ def lib() -> C.CDLL:
         "dsmc_mesh_load_raw": ([C.c_char_p, pF, U32, pU, U32, pF, pU, pU], I32),
         "dsmc_tri_box_overlap": ([pF, pF, pF, pF, pF], I32),
         "dsmc_cgrid_build":([I32, pF, U32, pU, U32, pF, U32, U32, U32, pF, pU, pU, pU, pU, pU, U32, U32, pU], I32),
-        "dsmc_op_mt_generate": ([P, P, P, P, U32, U32, P, P], I32),
+        "dsmc_batch_create": ([C.POINTER(P), U32, C.POINTER(P)], I32),
+        "dsmc_batch_destroy": ([P], None),
+        "dsmc_batch_last_error": ([P], C.c_char_p),
+        "dsmc_batch_run": ([P, U32], I32),
+        "dsmc_batch_set_sampling": ([P, I32, I32], I32),
+        "dsmc_batch_set_graphs": ([P, I32], I32),
+        "dsmc_batch_synchronize": ([P], I32),
+        "dsmc_op_mt_generate":([P, P, P, P, U32, U32, P, P], I32),
         "dsmc_op_cell_ids": ([pS, P, P, P, U32, P, P], I32),
         "dsmc_op_sort": ([P, U32, U32, P, P, P, P], I32),
     }

@@ -58,6 +58,12 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp), "-pthread"])
     os.replace(tmp, LIB)
+    # the reference-style command-line driver, linked against the library
+    app = PKG / "bin" / "dsmc_main"
+    app.parent.mkdir(parents=True, exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(PKG / "apps" / "dsmc_main.cpp"),
+          f"-L{LIB.parent}", "-ldsmc_hip", f"-Wl,-rpath,{LIB.parent}", "-Wl,-rpath,$ORIGIN/../lib",
+          "-o", str(app)])
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -22,18 +22,24 @@ void count_keys(const uint32_t* keys, uint32_t n, uint32_t* cnt, hipStream_t st)
 // (device) receives the total. scratch: kScanMaxTiles + 1 uint32.
 int scan(uint32_t* in, uint32_t n, uint32_t* out, uint32_t* cursor, bool clear_in, uint32_t* scratch,
          uint32_t* total_out, hipStream_t st);
-void scatter(const InstDesc* d, uint32_t par, uint32_t grid, hipStream_t st);
+// Step kernels take an array of n_inst descriptors (grid.y = instance).
+void scatter(const InstDesc* d, uint32_t par, uint32_t grid, hipStream_t st, uint32_t n_inst = 1);
+// Scan of the step: which = 0 cell counts, 1 injection face counts; max_n =
+// largest count array of the batch; scratch: n_inst * (kScanMaxTiles + 1).
+void step_scan(const InstDesc* d, uint32_t n_inst, uint32_t which, uint32_t max_n, uint32_t* scratch,
+               hipStream_t st);
 void scatter_keys(const uint32_t* keys, uint32_t n, uint32_t* cursor, uint32_t* perm, hipStream_t st);
 void fix_segments(const uint32_t* start, uint32_t n_cells, uint32_t* perm, hipStream_t st);
-void tile(uint32_t mode, const InstDesc* d, uint32_t par, uint32_t flags, uint32_t n_vwg, hipStream_t st);
+void tile(uint32_t mode, const InstDesc* d, uint32_t par, uint32_t flags, uint32_t n_vwg, hipStream_t st,
+          uint32_t n_inst = 1);
 void load_reference(const InstDesc* d, uint32_t par, uint32_t ppc, uint32_t n_threads, float sux, float suy,
                     float suz, float temp, hipStream_t st);
 void move_zero(const InstDesc* d, uint32_t par, uint32_t n, hipStream_t st);
 void load_count(const InstDesc* d, uint32_t n_streams, const float* fn, hipStream_t st);
 void load_generate(const InstDesc* d, uint32_t par, uint32_t n_streams, const float* fu, const float* fv,
                    const float* fw, const float* fT, hipStream_t st);
-void inject_count(const InstDesc* d, uint32_t n_threads, hipStream_t st);
-void inject_generate(const InstDesc* d, uint32_t opar, uint32_t n_threads, hipStream_t st);
+void inject_count(const InstDesc* d, uint32_t n_threads, hipStream_t st, uint32_t n_inst = 1);
+void inject_generate(const InstDesc* d, uint32_t opar, uint32_t n_threads, hipStream_t st, uint32_t n_inst = 1);
 void sample_cells(const InstDesc* d, uint32_t par, uint32_t mode, uint32_t ncells, hipStream_t st);
 void validate(const InstDesc* d, uint32_t par, uint32_t n, uint32_t* bad, hipStream_t st);
 

@@ -218,10 +218,111 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(uint32_t* in, uint3
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = *total;
 }
 
+// Step scans over a batch of instances (grid.y = instance). which = 0: the
+// cell counts (count -> cell_start, cursor, count cleared, total in ctr[1]);
+// which = 1: the injection face counts, in place (total in face_count[n]).
+// scratch: kScanMaxTiles + 1 words per instance.
+__device__ __forceinline__ void bscan_args(const InstDesc& I, uint32_t which, uint32_t*& in, uint32_t& n) {
+  in = which ? I.face_count : I.count;
+  n = which ? I.n_face_cells : I.ncells;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_bscan_reduce(const InstDesc* __restrict__ D, uint32_t which,
+                                                              uint32_t* scratch) {
+  const InstDesc& I = D[blockIdx.y];
+  uint32_t* in;
+  uint32_t n;
+  bscan_args(I, which, in, n);
+  if (blockIdx.x * kScanTile >= n) return;
+  __shared__ uint32_t red[kScanThreads / 64];
+  uint32_t base = blockIdx.x * kScanTile;
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    uint32_t i = base + k * kScanThreads + threadIdx.x;
+    if (i < n) sum += in[i];
+  }
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off);
+  if (lane_id() == 0) red[threadIdx.x / 64] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < kScanThreads / 64; ++w) t += red[w];
+    scratch[blockIdx.y * (dsmc_launch::kScanMaxTiles + 1) + blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_bscan_tiles(const InstDesc* __restrict__ D, uint32_t which,
+                                                             uint32_t* scratch) {
+  const InstDesc& I = D[blockIdx.y];
+  uint32_t* in;
+  uint32_t n;
+  bscan_args(I, which, in, n);
+  if (n == 0) return;
+  uint32_t* ts = scratch + blockIdx.y * (dsmc_launch::kScanMaxTiles + 1);
+  const uint32_t n_tiles = (n + kScanTile - 1) / kScanTile;
+  __shared__ uint32_t sh[kScanThreads / 64 + 1];
+  uint32_t v[kScanItems];
+  uint32_t local = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    uint32_t i = threadIdx.x * kScanItems + k;
+    v[k] = i < n_tiles ? ts[i] : 0u;
+    local += v[k];
+  }
+  uint32_t total;
+  uint32_t off = block_exclusive_scan(local, &total, sh);
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    uint32_t i = threadIdx.x * kScanItems + k;
+    if (i < n_tiles) ts[i] = off;
+    off += v[k];
+  }
+  if (threadIdx.x == 0) {
+    ts[dsmc_launch::kScanMaxTiles] = total;
+    if (which == 0) I.ctr[1] = total;
+  }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __restrict__ D, uint32_t which,
+                                                             const uint32_t* scratch) {
+  const InstDesc& I = D[blockIdx.y];
+  uint32_t* in;
+  uint32_t n;
+  bscan_args(I, which, in, n);
+  if (blockIdx.x * kScanTile >= n) return;
+  const uint32_t* ts = scratch + blockIdx.y * (dsmc_launch::kScanMaxTiles + 1);
+  uint32_t* out = which ? I.face_count : I.cell_start;
+  uint32_t* cursor = which ? nullptr : I.cursor;
+  __shared__ uint32_t sh[kScanThreads / 64 + 1];
+  uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t local = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    uint32_t i = base + k;
+    v[k] = i < n ? in[i] : 0u;
+    local += v[k];
+  }
+  uint32_t tot;
+  uint32_t off = block_exclusive_scan(local, &tot, sh) + ts[blockIdx.x];
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    uint32_t i = base + k;
+    if (i < n) {
+      out[i] = off;
+      if (cursor) cursor[i] = off;
+      if (which == 0) in[i] = 0u;
+    }
+    off += v[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = ts[dsmc_launch::kScanMaxTiles];
+}
+
 // =================================================================== scatter
 // perm[cursor[key[i]]++] = i with one atomic per run of equal keys in a wave.
 __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip, uint32_t par) {
-  const InstDesc& I = *Ip;
+  const InstDesc& I = Ip[blockIdx.y];
   const uint32_t n = I.ctr[0];
   const uint32_t* keys = I.key[par];
   const uint32_t lane = lane_id();
@@ -437,7 +538,8 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
 
 template <uint32_t MODE>
 __global__ void __launch_bounds__(kTileThreads) k_tile(const InstDesc* __restrict__ Ip, uint32_t par, uint32_t flags) {
-  const InstDesc& I = *Ip;
+  const InstDesc& I = Ip[blockIdx.y];
+  if (blockIdx.x >= I.n_vwg) return;  // batched launches size the grid for the largest instance
   __shared__ uint32_t s_start[kTileCells + 1];
   __shared__ uint32_t s_raw[kLdsCap];
   __shared__ uint32_t s_perm[kLdsCap];
@@ -458,7 +560,7 @@ __global__ void __launch_bounds__(kTileThreads) k_tile(const InstDesc* __restric
   (void)hits_acc;
   if (vw == 0 && tid == 0) I.ctr[0] = I.ctr[1];
 
-  for (uint32_t tile = vw; tile < I.ntiles; tile += gridDim.x) {
+  for (uint32_t tile = vw; tile < I.ntiles; tile += I.n_vwg) {
     const uint32_t c0 = tile * kTileCells;
     const uint32_t nc = min(kTileCells, I.ncells - c0);
     s_start[tid] = I.cell_start[c0 + min(tid, nc)];
@@ -664,7 +766,7 @@ __device__ __forceinline__ uint32_t face_of(const InstDesc& I, uint32_t g) {
 }
 
 __global__ void k_inject_count(const InstDesc* __restrict__ Ip) {
-  const InstDesc& I = *Ip;
+  const InstDesc& I = Ip[blockIdx.y];
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= I.n_streams) return;
   if (s >= I.n_face_cells) return;
@@ -679,7 +781,8 @@ __global__ void k_inject_count(const InstDesc* __restrict__ Ip) {
 }
 
 __global__ void k_inject_generate(const InstDesc* __restrict__ Ip, uint32_t opar) {
-  const InstDesc& I = *Ip;
+  const InstDesc& I = Ip[blockIdx.y];
+  if (I.n_face_cells == 0) return;
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t base = I.ctr[1];
   if (s == 0) {
@@ -849,8 +952,16 @@ int scan(uint32_t* in, uint32_t n, uint32_t* out, uint32_t* cursor, bool clear_i
   return 0;
 }
 
-void scatter(const InstDesc* d, uint32_t par, uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, d, par);
+void scatter(const InstDesc* d, uint32_t par, uint32_t grid, hipStream_t st, uint32_t n_inst) {
+  hipLaunchKernelGGL(k_scatter, dim3(grid, n_inst), dim3(256), 0, st, d, par);
+}
+
+void step_scan(const InstDesc* d, uint32_t n_inst, uint32_t which, uint32_t max_n, uint32_t* scratch,
+               hipStream_t st) {
+  uint32_t tiles = blocks_for(max_n > 0 ? max_n : 1u, kScanTile);
+  hipLaunchKernelGGL(k_bscan_reduce, dim3(tiles, n_inst), dim3(kScanThreads), 0, st, d, which, scratch);
+  hipLaunchKernelGGL(k_bscan_tiles, dim3(1, n_inst), dim3(kScanThreads), 0, st, d, which, scratch);
+  hipLaunchKernelGGL(k_bscan_apply, dim3(tiles, n_inst), dim3(kScanThreads), 0, st, d, which, scratch);
 }
 
 void scatter_keys(const uint32_t* keys, uint32_t n, uint32_t* cursor, uint32_t* perm, hipStream_t st) {
@@ -863,11 +974,12 @@ void fix_segments(const uint32_t* start, uint32_t n_cells, uint32_t* perm, hipSt
   hipLaunchKernelGGL(k_fix_segments, dim3(blocks_for(n_cells, 256)), dim3(256), 0, st, start, n_cells, perm);
 }
 
-void tile(uint32_t mode, const InstDesc* d, uint32_t par, uint32_t flags, uint32_t n_vwg, hipStream_t st) {
+void tile(uint32_t mode, const InstDesc* d, uint32_t par, uint32_t flags, uint32_t n_vwg, hipStream_t st,
+          uint32_t n_inst) {
   if (mode == kModeReference)
-    hipLaunchKernelGGL(k_tile<kModeReference>, dim3(n_vwg), dim3(kTileThreads), 0, st, d, par, flags);
+    hipLaunchKernelGGL(k_tile<kModeReference>, dim3(n_vwg, n_inst), dim3(kTileThreads), 0, st, d, par, flags);
   else
-    hipLaunchKernelGGL(k_tile<kModePhysics>, dim3(n_vwg), dim3(kTileThreads), 0, st, d, par, flags);
+    hipLaunchKernelGGL(k_tile<kModePhysics>, dim3(n_vwg, n_inst), dim3(kTileThreads), 0, st, d, par, flags);
 }
 
 void load_reference(const InstDesc* d, uint32_t par, uint32_t ppc, uint32_t n_threads, float sux, float suy,
@@ -891,14 +1003,14 @@ void load_generate(const InstDesc* d, uint32_t par, uint32_t n_streams, const fl
   hipLaunchKernelGGL(k_load_generate, dim3(blocks_for(n_streams, 256)), dim3(256), 0, st, d, par, fu, fv, fw, fT);
 }
 
-void inject_count(const InstDesc* d, uint32_t n_threads, hipStream_t st) {
+void inject_count(const InstDesc* d, uint32_t n_threads, hipStream_t st, uint32_t n_inst) {
   if (!n_threads) return;
-  hipLaunchKernelGGL(k_inject_count, dim3(blocks_for(n_threads, 256)), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_inject_count, dim3(blocks_for(n_threads, 256), n_inst), dim3(256), 0, st, d);
 }
 
-void inject_generate(const InstDesc* d, uint32_t opar, uint32_t n_threads, hipStream_t st) {
+void inject_generate(const InstDesc* d, uint32_t opar, uint32_t n_threads, hipStream_t st, uint32_t n_inst) {
   if (!n_threads) return;
-  hipLaunchKernelGGL(k_inject_generate, dim3(blocks_for(n_threads, 256)), dim3(256), 0, st, d, opar);
+  hipLaunchKernelGGL(k_inject_generate, dim3(blocks_for(n_threads, 256), n_inst), dim3(256), 0, st, d, opar);
 }
 
 void sample_cells(const InstDesc* d, uint32_t par, uint32_t mode, uint32_t ncells, hipStream_t st) {

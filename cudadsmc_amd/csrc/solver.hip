@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -60,10 +61,14 @@ struct dsmc_solver {
   uint32_t ppc = 0;
   float ntc_init = 0.0f;
   bool timing = false;
-  float t_ms[5] = {0, 0, 0, 0, 0};
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double t_ms[5] = {0, 0, 0, 0, 0};  // summed milliseconds per kernel class
+  uint64_t timed_steps = 0;
+  std::vector<hipEvent_t> ev_pool;   // 5 events per timed step, resolved lazily
+  size_t ev_used = 0;
   uint32_t debug_flags = 0;
   std::vector<uint32_t> rng_a, rng_b, rng_c;
+  std::map<uint64_t, hipGraphExec_t> graphs;  // (parity, flags, flags) -> two captured steps
+  bool use_graphs = true;
 
   int fail(int code, const char* fmt, ...) {
     char buf[1024];
@@ -97,8 +102,8 @@ struct dsmc_solver {
     if (stream) (void)hipStreamSynchronize(stream);
     free_geom();
     for (void* p : bufs) (void)hipFree(p);
-    for (auto& e : ev)
-      if (e) (void)hipEventDestroy(e);
+    for (auto& e : ev_pool) (void)hipEventDestroy(e);
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
     if (stream) (void)hipStreamDestroy(stream);
   }
   int upload_desc() { return hip(hipMemcpyAsync(d_desc, &h, sizeof(h), hipMemcpyHostToDevice, stream), "desc upload"); }
@@ -146,6 +151,8 @@ double inflow_flux(double n, double vm, double un) {
 uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1u) / m * m; }
 
 int step_once(dsmc_solver* h);
+uint32_t step_flags(const dsmc_solver* h, uint64_t stepno, bool* sample);
+int run_graph_pair(dsmc_solver* h, uint32_t f1, uint32_t f2);
 
 }  // namespace
 
@@ -581,9 +588,20 @@ int dsmc_step(dsmc_solver* h, float dt) {
 int dsmc_run(dsmc_solver* h, uint32_t steps, float dt) {
   if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
   if (!h->have_state) return h->fail(DSMC_ERR_STATE, "run before init_state");
-  if (h->timing)
-    for (auto& t : h->t_ms) t = 0.0f;
-  for (uint32_t i = 0; i < steps; ++i) CHECK_RC(dsmc_step(h, dt));
+  if (steps == 0) return DSMC_OK;
+  CHECK_RC(dsmc_step(h, dt));  // applies a dt change, then eager
+  uint32_t done = 1;
+  // pairs of steps replayed from captured graphs (timing needs per-step events)
+  while (h->use_graphs && !h->timing && steps - done >= 2) {
+    bool s1, s2;
+    uint32_t f1 = step_flags(h, h->steps + 1, &s1);
+    uint32_t f2 = step_flags(h, h->steps + 2, &s2);
+    CHECK_RC(run_graph_pair(h, f1, f2));
+    h->steps += 2;
+    h->samples += (s1 ? 1u : 0u) + (s2 ? 1u : 0u);
+    done += 2;
+  }
+  for (; done < steps; ++done) CHECK_RC(step_once(h));
   return DSMC_OK;
 }
 
@@ -951,18 +969,45 @@ int dsmc_get_collision_grid(dsmc_solver* h, uint32_t* flags, uint32_t n) {
   return DSMC_OK;
 }
 
+// Resolve the recorded per-step events into the kernel-class sums.
+static int fold_timings(dsmc_solver* h) {
+  if (h->ev_used == 0) return DSMC_OK;
+  HIPC(hipEventSynchronize(h->ev_pool[h->ev_used - 1]));
+  for (size_t g = 0; g + 5 <= h->ev_used; g += 5) {
+    float t;
+    for (int k = 0; k < 4; ++k) {
+      HIPC(hipEventElapsedTime(&t, h->ev_pool[g + k], h->ev_pool[g + k + 1]));
+      h->t_ms[k] += t;
+    }
+    HIPC(hipEventElapsedTime(&t, h->ev_pool[g], h->ev_pool[g + 4]));
+    h->t_ms[4] += t;
+    h->timed_steps += 1;
+  }
+  h->ev_used = 0;
+  return DSMC_OK;
+}
+
 int dsmc_get_timings(dsmc_solver* h, float* out_ms, uint32_t n) {
   if (!h || !out_ms) return set_global_error(DSMC_ERR_INVALID, "null argument");
-  for (uint32_t k = 0; k < n && k < 5; ++k) out_ms[k] = h->t_ms[k];
+  CHECK_RC(fold_timings(h));
+  double d = h->timed_steps ? (double)h->timed_steps : 1.0;
+  for (uint32_t k = 0; k < n && k < 5; ++k) out_ms[k] = (float)(h->t_ms[k] / d);
+  if (n > 5) out_ms[5] = (float)h->timed_steps;
   return DSMC_OK;
 }
 
 int dsmc_set_timing(dsmc_solver* h, int enabled) {
   if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  CHECK_RC(fold_timings(h));
   h->timing = enabled != 0;
-  if (h->timing && !h->ev[0])
-    for (auto& e : h->ev) HIPC(hipEventCreate(&e));
-  for (auto& t : h->t_ms) t = 0.0f;
+  for (auto& t : h->t_ms) t = 0.0;
+  h->timed_steps = 0;
+  return DSMC_OK;
+}
+
+int dsmc_set_graphs(dsmc_solver* h, int enabled) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  h->use_graphs = enabled != 0;
   return DSMC_OK;
 }
 
@@ -1075,43 +1120,57 @@ int dsmc_op_sort(const uint32_t* keys, uint32_t n, uint32_t n_cells, uint32_t* p
 
 namespace {
 
-int step_once(dsmc_solver* h) {
-  InstDesc& I = h->h;
-  hipStream_t st = h->stream;
-  const uint64_t stepno = h->steps + 1;
+// Flags of step number `stepno` (1-based): sampling cadence and debug outputs.
+uint32_t step_flags(const dsmc_solver* h, uint64_t stepno, bool* sample) {
   uint32_t flags = h->debug_flags;
-  bool sample = false;
-  if (h->physics()) sample = h->sampling;
+  bool smp;
+  if (h->physics()) smp = h->sampling;
   else {
     uint32_t every = h->s.sample_every ? h->s.sample_every : 100u;
-    sample = (stepno % every) == 0;
+    smp = (stepno % every) == 0;
   }
-  if (sample) flags |= dsmc_launch::kFlagSample;
-  if (h->timing) HIPC(hipEventRecord(h->ev[0], st));
-  if (dsmc_launch::scan(I.count, I.ncells, I.cell_start, I.cursor, true, h->scan_scratch, I.ctr + 1, st))
-    return h->fail(DSMC_ERR_CAPACITY, "too many cells to scan");
-  if (h->timing) HIPC(hipEventRecord(h->ev[1], st));
-  dsmc_launch::scatter(h->d_desc, h->par, h->scatter_grid, st);
-  if (h->timing) HIPC(hipEventRecord(h->ev[2], st));
-  dsmc_launch::tile(I.mode, h->d_desc, h->par, flags, I.n_vwg, st);
-  if (h->timing) HIPC(hipEventRecord(h->ev[3], st));
+  if (smp) flags |= dsmc_launch::kFlagSample;
+  if (sample) *sample = smp;
+  return flags;
+}
+
+// Enqueue the kernels of one step reading buffer `par` (no host sync, no
+// memsets: safe to capture in a HIP graph).
+int enqueue_step(dsmc_solver* h, uint32_t par, uint32_t flags, hipEvent_t* ev) {
+  InstDesc& I = h->h;
+  hipStream_t st = h->stream;
+  if (ev) HIPC(hipEventRecord(ev[0], st));
+  dsmc_launch::step_scan(h->d_desc, 1, 0, I.ncells, h->scan_scratch, st);
+  if (ev) HIPC(hipEventRecord(ev[1], st));
+  dsmc_launch::scatter(h->d_desc, par, h->scatter_grid, st);
+  if (ev) HIPC(hipEventRecord(ev[2], st));
+  dsmc_launch::tile(I.mode, h->d_desc, par, flags, I.n_vwg, st);
+  if (ev) HIPC(hipEventRecord(ev[3], st));
   if (I.n_face_cells) {
     dsmc_launch::inject_count(h->d_desc, std::min(I.n_streams, I.n_face_cells), st);
-    if (dsmc_launch::scan(I.face_count, I.n_face_cells, I.face_count, nullptr, false, h->scan_scratch, nullptr, st))
-      return h->fail(DSMC_ERR_CAPACITY, "too many face cells");
-    dsmc_launch::inject_generate(h->d_desc, h->par ^ 1u, std::min(I.n_streams, I.n_face_cells), st);
+    dsmc_launch::step_scan(h->d_desc, 1, 1, I.n_face_cells, h->scan_scratch, st);
+    dsmc_launch::inject_generate(h->d_desc, par ^ 1u, std::min(I.n_streams, I.n_face_cells), st);
   }
+  if (ev) HIPC(hipEventRecord(ev[4], st));
+  return DSMC_OK;
+}
+
+int step_once(dsmc_solver* h) {
+  const uint64_t stepno = h->steps + 1;
+  bool sample = false;
+  uint32_t flags = step_flags(h, stepno, &sample);
+  hipEvent_t* ev = nullptr;
   if (h->timing) {
-    HIPC(hipEventRecord(h->ev[4], st));
-    HIPC(hipEventSynchronize(h->ev[4]));
-    float t;
-    for (int k = 0; k < 4; ++k) {
-      HIPC(hipEventElapsedTime(&t, h->ev[k], h->ev[k + 1]));
-      h->t_ms[k] += t;
+    if (h->ev_used + 5 > 5 * 4096) CHECK_RC(fold_timings(h));
+    while (h->ev_pool.size() < h->ev_used + 5) {
+      hipEvent_t e;
+      HIPC(hipEventCreate(&e));
+      h->ev_pool.push_back(e);
     }
-    HIPC(hipEventElapsedTime(&t, h->ev[0], h->ev[4]));
-    h->t_ms[4] += t;
+    ev = &h->ev_pool[h->ev_used];
+    h->ev_used += 5;
   }
+  CHECK_RC(enqueue_step(h, h->par, flags, ev));
   HIPC(hipGetLastError());
   h->par ^= 1u;
   h->steps = stepno;
@@ -1119,4 +1178,204 @@ int step_once(dsmc_solver* h) {
   return DSMC_OK;
 }
 
+// Two consecutive steps (parity par, then par^1) captured as one graph.
+int run_graph_pair(dsmc_solver* h, uint32_t f1, uint32_t f2) {
+  uint64_t key = ((uint64_t)h->par << 63) | ((uint64_t)f1 << 32) | f2;
+  auto it = h->graphs.find(key);
+  hipGraphExec_t exec;
+  if (it == h->graphs.end()) {
+    hipGraph_t g;
+    HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc1 = enqueue_step(h, h->par, f1, nullptr);
+    int rc2 = rc1 == DSMC_OK ? enqueue_step(h, h->par ^ 1u, f2, nullptr) : rc1;
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc2 != DSMC_OK) return rc2;
+    HIPC(e);
+    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPC(e);
+    h->graphs[key] = exec;
+  } else {
+    exec = it->second;
+  }
+  HIPC(hipGraphLaunch(exec, h->stream));
+  return DSMC_OK;
+}
+
 }  // namespace
+
+// ------------------------------------------------------------------ batches
+// Several instances advanced by the same kernel launches (grid.y = instance):
+// the batched-ensemble step. Every instance keeps its own buffers, RNG
+// streams, dt and descriptor; the batch holds the descriptor array.
+struct dsmc_batch {
+  std::vector<dsmc_solver*> hs;
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
+  InstDesc* d_descs = nullptr;
+  uint32_t* scratch = nullptr;
+  uint32_t n = 0, mode = 0, max_cells = 0, max_faces = 0, max_vwg = 0, max_inj_threads = 0, scatter_grid = 0;
+  bool use_graphs = true;
+  std::map<uint64_t, hipGraphExec_t> graphs;
+  ~dsmc_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    if (d_descs) (void)hipFree(d_descs);
+    if (scratch) (void)hipFree(scratch);
+    if (ev) (void)hipEventDestroy(ev);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+};
+
+namespace {
+
+int batch_hip(dsmc_batch* b, hipError_t e, const char* what) {
+  if (e == hipSuccess) return DSMC_OK;
+  return b->fail(DSMC_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define BHIP(expr) CHECK_RC(batch_hip(b, (expr), #expr))
+
+int enqueue_batch_step(dsmc_batch* b, uint32_t par, uint32_t flags) {
+  hipStream_t st = b->stream;
+  dsmc_launch::step_scan(b->d_descs, b->n, 0, b->max_cells, b->scratch, st);
+  dsmc_launch::scatter(b->d_descs, par, b->scatter_grid, st, b->n);
+  dsmc_launch::tile(b->mode, b->d_descs, par, flags, b->max_vwg, st, b->n);
+  if (b->max_faces) {
+    dsmc_launch::inject_count(b->d_descs, b->max_inj_threads, st, b->n);
+    dsmc_launch::step_scan(b->d_descs, b->n, 1, b->max_faces, b->scratch, st);
+    dsmc_launch::inject_generate(b->d_descs, par ^ 1u, b->max_inj_threads, st, b->n);
+  }
+  return DSMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dsmc_batch_create(dsmc_solver* const* hs, uint32_t n, dsmc_batch** out) {
+  if (!hs || !out || n == 0) return set_global_error(DSMC_ERR_INVALID, "empty batch");
+  *out = nullptr;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!hs[i] || !hs[i]->have_state) return set_global_error(DSMC_ERR_STATE, "batch member without loaded state");
+    if (hs[i]->h.mode != hs[0]->h.mode) return set_global_error(DSMC_ERR_INVALID, "batch members differ in mode");
+    if (hs[i]->par != hs[0]->par || hs[i]->steps != hs[0]->steps)
+      return set_global_error(DSMC_ERR_STATE, "batch members must be at the same step");
+    for (uint32_t j = 0; j < i; ++j)
+      if (hs[j] == hs[i]) return set_global_error(DSMC_ERR_INVALID, "a solver appears twice in the batch");
+  }
+  auto* b = new dsmc_batch();
+  b->hs.assign(hs, hs + n);
+  b->n = n;
+  b->mode = hs[0]->h.mode;
+  for (auto* h : b->hs) {
+    b->max_cells = std::max(b->max_cells, h->h.ncells);
+    b->max_faces = std::max(b->max_faces, h->h.n_face_cells);
+    b->max_vwg = std::max(b->max_vwg, h->h.n_vwg);
+    b->max_inj_threads = std::max(b->max_inj_threads, std::min(h->h.n_streams, h->h.n_face_cells));
+    b->scatter_grid = std::max(b->scatter_grid, h->scatter_grid);
+  }
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipMalloc(&b->d_descs, sizeof(InstDesc) * n);
+  if (e == hipSuccess) e = hipMalloc(&b->scratch, sizeof(uint32_t) * n * (dsmc_launch::kScanMaxTiles + 1));
+  if (e != hipSuccess) {
+    delete b;
+    return set_global_error(DSMC_ERR_DEVICE, std::string("batch allocation: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return DSMC_OK;
+}
+
+void dsmc_batch_destroy(dsmc_batch* b) { delete b; }
+
+const char* dsmc_batch_last_error(const dsmc_batch* b) { return b ? b->err.c_str() : global_error(); }
+
+int dsmc_batch_set_graphs(dsmc_batch* b, int enabled) {
+  if (!b) return set_global_error(DSMC_ERR_INVALID, "null batch");
+  b->use_graphs = enabled != 0;
+  return DSMC_OK;
+}
+
+int dsmc_batch_set_sampling(dsmc_batch* b, int enabled, int reset) {
+  if (!b) return set_global_error(DSMC_ERR_INVALID, "null batch");
+  for (auto* h : b->hs) {
+    int rc = dsmc_set_sampling(h, enabled, reset);
+    if (rc) return b->fail(rc, h->err);
+  }
+  return DSMC_OK;
+}
+
+int dsmc_batch_run(dsmc_batch* b, uint32_t steps) {
+  if (!b) return set_global_error(DSMC_ERR_INVALID, "null batch");
+  dsmc_solver* h0 = b->hs[0];
+  for (auto* h : b->hs)
+    if (h->par != h0->par || h->steps != h0->steps) return b->fail(DSMC_ERR_STATE, "batch members drifted apart");
+  // order after each member's queued work, refresh the descriptors
+  for (auto* h : b->hs) {
+    BHIP(hipEventRecord(b->ev, h->stream));
+    BHIP(hipStreamWaitEvent(b->stream, b->ev, 0));
+  }
+  std::vector<InstDesc> descs(b->n);
+  for (uint32_t i = 0; i < b->n; ++i) descs[i] = b->hs[i]->h;
+  BHIP(hipMemcpyAsync(b->d_descs, descs.data(), sizeof(InstDesc) * b->n, hipMemcpyHostToDevice, b->stream));
+  uint32_t done = 0;
+  uint32_t par = h0->par;
+  uint64_t stepno = h0->steps;
+  uint32_t samples = 0;
+  while (done < steps) {
+    bool s1 = false, s2 = false;
+    uint32_t f1 = step_flags(h0, stepno + 1, &s1);
+    if (b->use_graphs && steps - done >= 2) {
+      uint32_t f2 = step_flags(h0, stepno + 2, &s2);
+      uint64_t key = ((uint64_t)par << 63) | ((uint64_t)f1 << 32) | f2;
+      auto it = b->graphs.find(key);
+      hipGraphExec_t exec;
+      if (it == b->graphs.end()) {
+        hipGraph_t g;
+        BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_batch_step(b, par, f1);
+        enqueue_batch_step(b, par ^ 1u, f2);
+        BHIP(hipStreamEndCapture(b->stream, &g));
+        hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        BHIP(e);
+        b->graphs[key] = exec;
+      } else {
+        exec = it->second;
+      }
+      BHIP(hipGraphLaunch(exec, b->stream));
+      stepno += 2;
+      done += 2;
+      samples += (s1 ? 1u : 0u) + (s2 ? 1u : 0u);
+    } else {
+      CHECK_RC(enqueue_batch_step(b, par, f1));
+      par ^= 1u;
+      stepno += 1;
+      done += 1;
+      samples += s1 ? 1u : 0u;
+    }
+  }
+  BHIP(hipGetLastError());
+  for (auto* h : b->hs) {
+    h->par = par;
+    h->steps = stepno;
+    h->samples += samples;
+  }
+  // members' later calls run after the batch's work
+  BHIP(hipEventRecord(b->ev, b->stream));
+  for (auto* h : b->hs) BHIP(hipStreamWaitEvent(h->stream, b->ev, 0));
+  return DSMC_OK;
+}
+
+int dsmc_batch_synchronize(dsmc_batch* b) {
+  if (!b) return set_global_error(DSMC_ERR_INVALID, "null batch");
+  BHIP(hipStreamSynchronize(b->stream));
+  return DSMC_OK;
+}
+
+}  // extern "C"

@@ -205,11 +205,17 @@ int dsmc_get_fluid_fraction(dsmc_solver* h, float* out, uint32_t n_cells);
 /* Collision grid of the geometry: flags (0 empty, 1 triangles, 2 filled
  * interior) for gx*gy*gz cells. */
 int dsmc_get_collision_grid(dsmc_solver* h, uint32_t* flags, uint32_t n);
-/* Per-step kernel timings of the last dsmc_run (milliseconds per kernel
- * class: scan, scatter, cells, inject, total). */
+/* Average milliseconds per step of each kernel class {scan, scatter, cells,
+ * inject, total} over the steps timed since dsmc_set_timing; out_ms[5]
+ * (when n > 5) receives the number of timed steps. */
 int dsmc_get_timings(dsmc_solver* h, float* out_ms, uint32_t n);
-/* Enable hipEvent timing of each kernel class in dsmc_run (off by default). */
+/* Enable hipEvent timing of each kernel class (off by default). Events are
+ * recorded on the solver's stream without host synchronisation and resolved
+ * by dsmc_get_timings; enabling resets the averages. */
 int dsmc_set_timing(dsmc_solver* h, int enabled);
+/* dsmc_run replays pairs of steps from captured HIP graphs (default on);
+ * off: every step is launched eagerly. Results are identical either way. */
+int dsmc_set_graphs(dsmc_solver* h, int enabled);
 /* Debug outputs of each step: bit 2 keeps the sorted permutation readable
  * by dsmc_get_perm, bit 4 records wall hits for dsmc_get_hit_counts. */
 int dsmc_set_debug(dsmc_solver* h, uint32_t flags);
@@ -250,6 +256,22 @@ int dsmc_cgrid_build(int mode, const float* verts, uint32_t nv, const uint32_t* 
                      const float* normals, uint32_t gx, uint32_t gy, uint32_t gz, float bounds[9],
                      uint32_t dims[3], uint32_t* flags, uint32_t* start, uint32_t* end, uint32_t* entries,
                      uint32_t cap_cells, uint32_t cap_entries, uint32_t* n_entries);
+
+/* ------------------------------------------------ batched ensembles (C4) */
+/* Several independent instances (each its own handle: grid, particle count,
+ * freestream, body, seeds, dt) advanced by the same kernel launches, one
+ * launch per step kernel for the whole ensemble (grid.y = instance), with
+ * pairs of steps replayed from captured HIP graphs. Members must be loaded
+ * and at the same step; results are identical to stepping each member alone.
+ * The batch form of the reference's DoBenchmark loop (dsmc_base.cu:64-84). */
+typedef struct dsmc_batch dsmc_batch;
+int dsmc_batch_create(dsmc_solver* const* solvers, uint32_t n, dsmc_batch** out);
+void dsmc_batch_destroy(dsmc_batch* b);
+const char* dsmc_batch_last_error(const dsmc_batch* b);
+int dsmc_batch_run(dsmc_batch* b, uint32_t steps);
+int dsmc_batch_set_sampling(dsmc_batch* b, int enabled, int reset);
+int dsmc_batch_set_graphs(dsmc_batch* b, int enabled);
+int dsmc_batch_synchronize(dsmc_batch* b);
 
 /* --------------------------------------- kernel-level operators (d_ = device) */
 /* MT(19,9) streams: seeds[s] with parameters (a[s], b[s], c[s]); writes

@@ -268,22 +268,52 @@ struct oracle_sim {
   }
 
   // --------------------------------------------------- PHYSICS move
+  // Nearest front-facing hit with t in [0, dist]; ties on t go to the lower
+  // triangle index. Small meshes: every triangle. Large meshes: the triangles
+  // listed in the collision cells overlapping the segment's bounding box
+  // (any triangle the segment hits touches the cell holding the hit point,
+  // which lies in that box).
+  bool test_tri(uint32_t tri, float ox, float oy, float oz, float dx, float dy, float dz, float dist, uint32_t skip,
+                bool& found, float& best, uint32_t& bt) const {
+    if (tri == skip) return false;
+    const V3& n = tn[tri];
+    if (!(dx * n.x + dy * n.y + dz * n.z < 0.0f)) return false;
+    float t, u, v;
+    if (mt_intersect(ox, oy, oz, dx, dy, dz, tv0[tri], tv1[tri], tv2[tri], t, u, v) && t >= 0.0f && t <= dist) {
+      if (!found || t < best || (t == best && tri < bt)) {
+        found = true;
+        best = t;
+        bt = tri;
+      }
+    }
+    return true;
+  }
+
   bool nearest_hit(float ox, float oy, float oz, float dx, float dy, float dz, float dist, uint32_t skip, float& best,
                    uint32_t& bt) const {
     bool found = false;
-    for (uint32_t tri = 0; tri < (uint32_t)tn.size(); ++tri) {
-      if (tri == skip) continue;
-      const V3& n = tn[tri];
-      if (!(dx * n.x + dy * n.y + dz * n.z < 0.0f)) continue;
-      float t, u, v;
-      if (mt_intersect(ox, oy, oz, dx, dy, dz, tv0[tri], tv1[tri], tv2[tri], t, u, v) && t >= 0.0f && t <= dist) {
-        if (!found || t < best) {  // ascending tri: ties keep the lower index
-          found = true;
-          best = t;
-          bt = tri;
-        }
-      }
+    if (tn.size() <= 4096) {
+      for (uint32_t tri = 0; tri < (uint32_t)tn.size(); ++tri)
+        test_tri(tri, ox, oy, oz, dx, dy, dz, dist, skip, found, best, bt);
+      return found;
     }
+    float ex = ox + dx * dist, ey = oy + dy * dist, ez = oz + dz * dist;
+    float lo[3] = {std::min(ox, ex), std::min(oy, ey), std::min(oz, ez)};
+    float hi[3] = {std::max(ox, ex), std::max(oy, ey), std::max(oz, ez)};
+    int a[3], b[3];
+    for (int k = 0; k < 3; ++k) {
+      if (hi[k] < cgmin[k] || lo[k] > cgmax[k]) return false;
+      a[k] = std::max(0, (int)std::floor((lo[k] - cgmin[k]) / cgcell[k]) - 1);
+      b[k] = std::min((int)cgdim[k] - 1, (int)std::floor((hi[k] - cgmin[k]) / cgcell[k]) + 1);
+    }
+    for (int z = a[2]; z <= b[2]; ++z)
+      for (int y = a[1]; y <= b[1]; ++y)
+        for (int x = a[0]; x <= b[0]; ++x) {
+          uint32_t c = (uint32_t)z * cgdim[0] * cgdim[1] + (uint32_t)y * cgdim[0] + (uint32_t)x;
+          if (cgflag[c] != 1) continue;
+          for (uint32_t i = cgstart[c]; i < cgend[c]; ++i)
+            test_tri(cgtri[i], ox, oy, oz, dx, dy, dz, dist, skip, found, best, bt);
+        }
     return found;
   }
 
