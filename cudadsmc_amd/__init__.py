@@ -295,6 +295,9 @@ class Solver:
     def set_graphs(self, enabled: bool) -> None:
         self._c(lib().dsmc_set_graphs(self._h, int(enabled)))
 
+    def set_scatter_grid(self, blocks: int) -> None:
+        self._c(lib().dsmc_set_scatter_grid(self._h, int(blocks)))
+
     def set_timing(self, enabled: bool) -> None:
         self._c(lib().dsmc_set_timing(self._h, int(enabled)))
 

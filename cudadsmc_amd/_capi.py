@@ -105,6 +105,7 @@ def lib() -> C.CDLL:
         "dsmc_set_timing": ([P, I32], I32),
         "dsmc_set_debug": ([P, U32], I32),
         "dsmc_set_graphs": ([P, I32], I32),
+        "dsmc_set_scatter_grid": ([P, U32], I32),
         "dsmc_get_layout": ([P, pU, pU, pU, pU], I32),
         "dsmc_get_derived": ([P, pF, U32], I32),
         "dsmc_get_rng_params": ([P, pU, pU, pU, U32], I32),

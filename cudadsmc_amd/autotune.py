@@ -1,0 +1,102 @@
+"""Autotuner of the DSMC step's launch layout.
+
+Two parameters are tuned by timing the real case on the GPU:
+
+* ``n_streams``: the number of RNG streams, which fixes the tile kernel's
+  grid (n_streams / 256 workgroups, each walking tiles vwg, vwg + G, ...).
+  More workgroups give more parallelism; fewer give longer tile walks and
+  less RNG-state traffic. It fixes the stream assignment, so it is chosen
+  before particles are loaded and must be recorded with the results (a run
+  is reproducible for a given value).
+* ``scatter_grid``: workgroups of the grid-stride scatter kernel (does not
+  change results).
+
+The tuner builds one solver per candidate (``make(n_streams) -> Solver``,
+loaded and ready), warms it up, times steps between device
+synchronisations, and keeps the fastest. Results can be cached in a JSON
+file keyed by a case signature.
+"""
+from __future__ import annotations
+
+import json
+import time
+from pathlib import Path
+from typing import Callable, Iterable
+
+DEFAULT_STREAMS = (16384, 32768, 65536, 131072, 262144, 524288)
+DEFAULT_SCATTER = (1024, 2048, 4096, 8192, 16384)
+
+
+def _time_steps(run: Callable[[int], None], sync: Callable[[], None], warmup: int, steps: int) -> float:
+    run(warmup)
+    sync()
+    t0 = time.perf_counter()
+    run(steps)
+    sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def tune_solver(make: Callable[[int], "object"], streams: Iterable[int] = DEFAULT_STREAMS,
+                scatter: Iterable[int] = DEFAULT_SCATTER, warmup: int = 20, steps: int = 40,
+                cache: str | Path | None = None, key: str | None = None) -> dict:
+    """Best {n_streams, scatter_grid} for one solver case."""
+    if cache and key:
+        db = _load(cache)
+        if key in db:
+            return db[key]
+    table = []
+    best = None
+    for ns in streams:
+        sol = make(int(ns))
+        dt = sol.settings.dt
+        t = _time_steps(lambda k: sol.DoBenchmark(k, dt), sol.synchronize, warmup, steps)
+        table.append({"n_streams": int(ns), "scatter_grid": None, "ms_per_step": 1e3 * t})
+        if best is None or t < best[0]:
+            best = (t, int(ns))
+        sol.close()
+    sol = make(best[1])
+    dt = sol.settings.dt
+    sbest = None
+    for g in scatter:
+        sol.set_scatter_grid(int(g))
+        t = _time_steps(lambda k: sol.DoBenchmark(k, dt), sol.synchronize, warmup // 2 + 2, steps)
+        table.append({"n_streams": best[1], "scatter_grid": int(g), "ms_per_step": 1e3 * t})
+        if sbest is None or t < sbest[0]:
+            sbest = (t, int(g))
+    sol.close()
+    out = {"n_streams": best[1], "scatter_grid": sbest[1], "ms_per_step": 1e3 * sbest[0], "table": table}
+    if cache and key:
+        db = _load(cache)
+        db[key] = out
+        Path(cache).write_text(json.dumps(db, indent=1))
+    return out
+
+
+def tune_batch(make_members: Callable[[int], list], streams: Iterable[int] = (4096, 8192, 16384, 32768, 65536),
+               warmup: int = 10, steps: int = 20) -> dict:
+    """Best per-instance n_streams for an ensemble stepped as one batch."""
+    import cudadsmc_amd as dm
+
+    table = []
+    best = None
+    for ns in streams:
+        members = make_members(int(ns))
+        b = dm.Batch(members)
+        t = _time_steps(b.run, b.synchronize, warmup, steps)
+        table.append({"n_streams": int(ns), "ms_per_step": 1e3 * t})
+        if best is None or t < best[0]:
+            best = (t, int(ns))
+        b.close()
+        for m in members:
+            m.close()
+    return {"n_streams": best[1], "ms_per_step": 1e3 * best[0], "table": table}
+
+
+def _load(path) -> dict:
+    p = Path(path)
+    if p.exists():
+        try:
+            return json.loads(p.read_text())
+        except json.JSONDecodeError:
+            return {}
+    return {}

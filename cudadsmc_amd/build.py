@@ -20,7 +20,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "lib" / "libdsmc_hip.so"
 
 HOST_SOURCES = ["host/settings.cpp", "host/geometry.cpp", "host/mt_table.cpp", "host/capi_host.cpp"]
-HIP_SOURCES = ["kernels/step.hip", "solver.hip"]
+HIP_SOURCES = ["kernels/step.hip", "kernels/no.hip", "solver.hip"]
 
 ARCH = os.environ.get("DSMC_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

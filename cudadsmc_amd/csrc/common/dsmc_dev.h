@@ -37,6 +37,16 @@ constexpr float kModelWidth = 0.0001f;            // MODEL_WIDTH, dsmc_kernel.cu
 constexpr int kCollisionLimit = 10;               // COLLISION_LIMIT, dsmc_kernel.cu:1250
 constexpr double kTriEpsilon = 0.000001;          // EPSILON, tri_intersect.cu:1
 constexpr int kPhysMaxBounces = 8;
+// Smallest fluid fraction used for a cell's NTC volume: the fraction is
+// sampled at 4^3 points, so a cell reported empty may still hold a sliver of
+// fluid that particles reach; its volume is then taken as one sample's share.
+constexpr float kMinFluidFrac = 1.0f / 64.0f;
+// Candidate pairs allowed per cell and step. A larger count only arises from
+// a corrupted state (non-finite (sigma c_r)max); the count is clamped and
+// status bit kStatusPairs is raised instead of spinning for ~2^32 pairs.
+constexpr uint32_t kMaxPairsPerCell = 1u << 20;
+// Bits of the sticky status word ctr[3].
+constexpr uint32_t kStatusCapacity = 1u, kStatusPairs = 2u;
 
 // Collision-grid cell flags.
 constexpr uint32_t kCgEmpty = 0, kCgSurface = 1, kCgFilled = 2;

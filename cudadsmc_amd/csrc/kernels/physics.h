@@ -441,6 +441,10 @@ __device__ __forceinline__ void ntc_reference(const InstDesc& I, float* vx, floa
   uint32_t pairs = dm_f2u_sat(fpc);
   st.x += (float)pairs;
   nt.y = fpc - (float)pairs;
+  if (pairs > kMaxPairsPerCell) {
+    pairs = kMaxPairsPerCell;
+    atomicOr(&I.ctr[3], kStatusPairs);
+  }
   for (uint32_t k = 0; k < pairs; ++k) {
     uint32_t i = mt_next(rng) % (uint32_t)part;
     uint32_t j = mt_next(rng) % (uint32_t)part;
@@ -469,6 +473,10 @@ __device__ __forceinline__ void ntc_physics(const InstDesc& I, float* vx, float*
                           (double)vol + (double)nt.y);
   uint32_t pairs = dm_f2u_sat(fpc);
   nt.y = fpc - (float)pairs;
+  if (pairs > kMaxPairsPerCell) {
+    pairs = kMaxPairsPerCell;
+    atomicOr(&I.ctr[3], kStatusPairs);
+  }
   acc.sel += (double)pairs;
   for (uint32_t k = 0; k < pairs; ++k) {
     uint32_t i = mt_next(rng) % n;

@@ -455,7 +455,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
         ntc_reference(I, vx + a, vy + a, vz + a, n, nt, st, rng);
         I.cstat[c] = st;
       } else {
-        float vol = I.cell_volume * (I.fluid_frac ? I.fluid_frac[c] : 1.0f);
+        float vol = I.cell_volume * (I.fluid_frac ? fmaxf(I.fluid_frac[c], kMinFluidFrac) : 1.0f);
         ntc_physics(I, vx + a, vy + a, vz + a, n, vol, nt, rng, cacc);
       }
       I.ntc[c] = nt;
@@ -790,7 +790,7 @@ __global__ void k_inject_generate(const InstDesc* __restrict__ Ip, uint32_t opar
     uint32_t end = base + tot;
     if (end > I.cap || end < base) {  // capacity exceeded: drop the excess, flag it
       end = I.cap;
-      I.ctr[3] = 1u;
+      atomicOr(&I.ctr[3], kStatusCapacity);
     }
     I.ctr[0] = end;
     I.ctr[2] += end - base;

@@ -114,6 +114,16 @@ struct dsmc_solver {
     return v;
   }
   bool physics() const { return s.mode == DSMC_MODE_PHYSICS; }
+  // Sticky device status word (ctr[3]), read at synchronising calls.
+  int check_status() {
+    uint32_t st = get_ctr(3);
+    if (st & kStatusCapacity)
+      return fail(DSMC_ERR_CAPACITY, "particle capacity %u exceeded: injected particles were dropped", h.cap);
+    if (st & kStatusPairs)
+      return fail(DSMC_ERR_NUMERIC, "more than %u NTC candidate pairs in one cell (non-finite collision state)",
+                  kMaxPairsPerCell);
+    return DSMC_OK;
+  }
 };
 
 #define CHECK_RC(expr)          \
@@ -828,7 +838,7 @@ int dsmc_get_counts(dsmc_solver* h, uint64_t* steps, uint32_t* n_particles, uint
 int dsmc_synchronize(dsmc_solver* h) {
   if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
   HIPC(hipStreamSynchronize(h->stream));
-  return DSMC_OK;
+  return h->inited ? h->check_status() : DSMC_OK;
 }
 
 int dsmc_validate(dsmc_solver* h, uint32_t* n_bad) {
@@ -841,6 +851,7 @@ int dsmc_validate(dsmc_solver* h, uint32_t* n_bad) {
   HIPC(hipGetLastError());
   uint32_t b = h->get_ctr(8);
   if (n_bad) *n_bad = b;
+  CHECK_RC(h->check_status());
   return b ? h->fail(DSMC_ERR_NUMERIC, "%u particles with non-finite state or outside the box", b) : DSMC_OK;
 }
 
@@ -1002,6 +1013,16 @@ int dsmc_set_timing(dsmc_solver* h, int enabled) {
   h->timing = enabled != 0;
   for (auto& t : h->t_ms) t = 0.0;
   h->timed_steps = 0;
+  return DSMC_OK;
+}
+
+int dsmc_set_scatter_grid(dsmc_solver* h, uint32_t blocks) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->inited) return h->fail(DSMC_ERR_STATE, "not initialised");
+  if (blocks == 0 || blocks > 65535) return h->fail(DSMC_ERR_INVALID, "scatter grid must be in [1, 65535]");
+  h->scatter_grid = blocks;
+  for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.second);
+  h->graphs.clear();
   return DSMC_OK;
 }
 
@@ -1375,6 +1396,10 @@ int dsmc_batch_run(dsmc_batch* b, uint32_t steps) {
 int dsmc_batch_synchronize(dsmc_batch* b) {
   if (!b) return set_global_error(DSMC_ERR_INVALID, "null batch");
   BHIP(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < b->hs.size(); ++i) {
+    int rc = b->hs[i]->check_status();
+    if (rc != DSMC_OK) return b->fail(rc, "member " + std::to_string(i) + ": " + b->hs[i]->err);
+  }
   return DSMC_OK;
 }
 

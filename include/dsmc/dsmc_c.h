@@ -213,6 +213,8 @@ int dsmc_get_timings(dsmc_solver* h, float* out_ms, uint32_t n);
  * recorded on the solver's stream without host synchronisation and resolved
  * by dsmc_get_timings; enabling resets the averages. */
 int dsmc_set_timing(dsmc_solver* h, int enabled);
+/* Workgroups of the scatter kernel (grid-stride; does not change results). */
+int dsmc_set_scatter_grid(dsmc_solver* h, uint32_t blocks);
 /* dsmc_run replays pairs of steps from captured HIP graphs (default on);
  * off: every step is launched eagerly. Results are identical either way. */
 int dsmc_set_graphs(dsmc_solver* h, int enabled);
@@ -289,6 +291,58 @@ int dsmc_op_cell_ids(const dsmc_settings* s, const float* d_x, const float* d_y,
  * d_work must hold 2*n_cells + 4097 uint32. */
 int dsmc_op_sort(const uint32_t* d_keys, uint32_t n, uint32_t n_cells, uint32_t* d_perm,
                  uint32_t* d_cell_start, uint32_t* d_work, void* stream);
+
+/* ------------------------------------ field operator (warm-start loading) */
+/* The reference has no counterpart: these kernels train and evaluate the 3-D
+ * convolutional operator that predicts steady fields (n, u, v, w, T) on the
+ * cell grid from the case parameters; dsmc_init_state_from_fields then loads
+ * particles from the prediction instead of the uniform freestream of
+ * GenerateInitialVelocities (kernels/dsmc_kernel.cu:629-680).
+ * Layouts: activations BF16 channels-last [B][Z][Y][X][C]; weights BF16
+ * [taps][Cout][Cin] (taps 27 = 3x3x3 with dilation dil and zero padding, or
+ * 1); biases, gradients and the optimiser state FP32. All pointers are
+ * device pointers; calls are asynchronous on `stream`. */
+
+/* y = act(conv(x, w) + bias) (+ res): relu != 0 applies ReLU before the
+ * residual add; h (optional) receives the activation before the residual;
+ * y (BF16) and/or y32 (FP32) receive the result. Cin % 16 == 0,
+ * Cout % 32 == 0, Cout <= 128. */
+int dsmc_no_conv_fwd(const void* x, const void* w, const float* bias, const void* res, void* y, float* y32,
+                     void* h, int B, int Z, int Y, int X, int Cin, int Cout, int taps, int dil, int relu,
+                     void* stream);
+/* wb[mirror(t)][ci][co] = w[t][co][ci]: conv_fwd with wb computes the data
+ * gradient of conv_fwd with w. */
+int dsmc_no_weight_flip(const void* w, void* wb, int taps, int Cout, int Cin, void* stream);
+/* dw[t][co][ci] (+)= sum over voxels m of dy[m][co] * x[m + offset(t)][ci],
+ * through per-chunk partials (part: ceil(M / chunk) * taps * Cout * Cin
+ * floats) reduced in a fixed order. chunk % 64 == 0; Cin, Cout multiples of
+ * 32, at most 128. */
+int dsmc_no_conv_bwd_w(const void* x, const void* dy, float* dw, float* part, long part_floats, int B, int Z,
+                       int Y, int X, int Cin, int Cout, int taps, int dil, int chunk, int accumulate, void* stream);
+/* db[co] (+)= sum over voxels of dy[m][co]. */
+int dsmc_no_bias_grad(const void* dy, long M, int Cout, float* db, float* part, long part_floats, int accumulate,
+                      void* stream);
+/* dz = dy where h > 0, else 0. */
+int dsmc_no_relu_bwd(const void* dy, const void* h, void* dz, long n, void* stream);
+/* c = a + b (BF16). */
+int dsmc_no_add(const void* a, const void* b, void* c, long n, void* stream);
+/* Masked mean-squared error of pred[M][C] (first Cf channels) against
+ * target[M][Cf]: dpred (BF16 [M][C]) = 2 (pred - target) mask * inv_count,
+ * loss_part[block] = partial sums of (pred - target)^2 mask * inv_count over
+ * blocks of 256 elements. */
+int dsmc_no_mse(const float* pred, const float* target, const float* mask, long M, int C, int Cf, float inv_count,
+                void* dpred, float* loss_part, void* stream);
+/* Adam with bias corrections c1 = 1 - b1^t, c2 = 1 - b2^t and L2
+ * weight decay wd; refreshes the BF16 copy wbf when given. */
+int dsmc_no_adam(float* w, const float* g, float* m1, float* m2, void* wbf, long n, float lr, float b1, float b2,
+                 float eps, float c1, float c2, float wd, void* stream);
+int dsmc_no_f32_to_bf16(const float* a, void* b, long n, void* stream);
+/* Operator inputs [B][Z][Y][X][C]: cell-centre coordinates in [-1, 1],
+ * signed distance to the sphere sphere[4b..4b+3] = (cx, cy, cz, r) in units
+ * of the box height, and the five case parameters p[5b..5b+4]; channels 9..C-1
+ * zero. box = (xmin, ymin, zmin, xmax, ymax, zmax). */
+int dsmc_no_inputs(const float* p, const float* sphere, const float* box, int B, int Z, int Y, int X, int C,
+                   void* out, void* stream);
 
 #ifdef __cplusplus
 }

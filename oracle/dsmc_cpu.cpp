@@ -43,6 +43,8 @@ const uint32_t INV = 0xFFFFFFFFu;
 const float BOLZ_REF = 1.3806e-23f;
 const double BOLZ = 1.380649e-23;
 const float PI_REF = 3.14159265358979f;
+// candidate pairs per cell and step beyond which the engine clamps and flags
+const uint32_t kMaxPairs = 1u << 20;
 
 // ------------------------------------------------------------------ MT
 struct MtBlock {  // genrand_mt-style generator (own implementation)
@@ -430,6 +432,7 @@ struct oracle_sim {
     uint32_t pairs = dm_f2u_sat(fpc);
     st[0] += pairs;
     nt[1] = fpc - pairs;
+    pairs = std::min(pairs, kMaxPairs);
     for (uint32_t k = 0; k < pairs; ++k) {
       uint32_t i = r.next() % (uint32_t)part;
       uint32_t j = r.next() % (uint32_t)part;
@@ -451,6 +454,7 @@ struct oracle_sim {
                         (double)nt[1]);
     uint32_t pairs = dm_f2u_sat(fpc);
     nt[1] = fpc - (float)pairs;
+    pairs = std::min(pairs, kMaxPairs);
     a[0] += pairs;
     for (uint32_t k = 0; k < pairs; ++k) {
       uint32_t i = r.next() % n;
@@ -548,7 +552,7 @@ struct oracle_sim {
           uint32_t a0 = cell_start[c], b0 = cell_start[c + 1];
           MtBlock& r = rng[(size_t)vw * TT + l];
           if (phys) {
-            float vol = cell_volume * (frac.empty() ? 1.0f : frac[c]);
+            float vol = cell_volume * (frac.empty() ? 1.0f : std::max(frac[c], 1.0f / 64.0f));
             ntc_physics(&nvx[a0], &nvy[a0], &nvz[a0], b0 - a0, vol, &ntc[2 * (size_t)c], r, a);
           } else {
             ntc_reference(&nvx[a0], &nvy[a0], &nvz[a0], (int)(b0 - a0), &ntc[2 * (size_t)c], &cstat[4 * (size_t)c], r);
