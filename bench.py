@@ -113,6 +113,43 @@ def cpu_baseline(seconds: float = 15.0) -> dict:
             "cpu_model": _cpu_model()}
 
 
+def ensemble_bench(steps: int, warmup: int, instances: int = 64) -> dict:
+    """The batched ensemble: `instances` argon sphere cases on 64x32x32 grids,
+    particle counts log-uniform in [1e4, 5e5], U in [600, 3000] m/s, n in
+    [3e19, 1e21] m^-3, r in [0.015, 0.035] m, seeds 0..instances-1, advanced
+    by the batched step with HIP graph replay; RNG streams per instance from
+    the autotuner's table."""
+    import cudadsmc_amd as dm
+    from cudadsmc_amd import ensemble as ens
+
+    ns = int(ens.tuned("ensemble64", 65536))
+    cases = ens.draw_cases(instances, seed=2024, temperatures=False)
+    t0 = time.perf_counter()
+    sols = [ens.make_solver(c, n_streams=ns) for c in cases]
+    b = dm.Batch(sols)
+    b.synchronize()
+    setup = time.perf_counter() - t0
+    b.run(warmup)
+    b.synchronize()
+    n0 = sum(s.counts()[1] for s in sols)
+    t0 = time.perf_counter()
+    b.run(steps)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    n1 = sum(s.counts()[1] for s in sols)
+    navg = 0.5 * (n0 + n1)
+    cells = instances * int(np.prod(ens.GRID))
+    b.close()
+    for s in sols:
+        s.close()
+    ps = navg * steps / el
+    return {"instances": instances, "particles_total": navg, "cells_total": cells, "steps": steps,
+            "warmup": warmup, "ms_per_step": 1e3 * el / steps, "particle_steps_per_s": ps,
+            "pct_hbm_peak": 100.0 * (B_P * navg + B_C * cells) * steps / el / HBM_PEAK,
+            "n_streams_per_instance": ns, "setup_s": setup,
+            "N_range": [min(c.N for c in cases), max(c.N for c in cases)]}
+
+
 def _cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -131,6 +168,8 @@ def main() -> int:
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition is "
@@ -211,6 +250,11 @@ def main() -> int:
                      "kernel": "k_tile (56 B/particle + 20 B/cell algorithmic)"},
         "validate_bad_particles": int(bad),
     }
+    sol.close()
+    if not args.no_ensemble:
+        t0 = time.perf_counter()
+        line["ensemble"] = ensemble_bench(args.ensemble_steps, 100)
+        print(f"[bench] ensemble {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

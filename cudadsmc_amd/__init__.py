@@ -260,6 +260,14 @@ class Solver:
         w = np.ascontiguousarray(words, np.uint32).ravel()
         self._c(lib().dsmc_set_rng_state(self._h, _up(w), len(w)))
 
+    def taus_state(self) -> np.ndarray:
+        """Hybrid-Taus collision RNG state (REFERENCE mode with mt = 0), (cells, 4)."""
+        return self._u32(lib().dsmc_get_taus_state, 4 * self.n_cells).reshape(self.n_cells, 4)
+
+    def set_taus_state(self, words: np.ndarray) -> None:
+        w = np.ascontiguousarray(words, np.uint32).ravel()
+        self._c(lib().dsmc_set_taus_state(self._h, _up(w), len(w)))
+
     def ntc_state(self) -> np.ndarray:
         a = np.zeros(2 * self.n_cells, np.float32)
         self._c(lib().dsmc_get_ntc_state(self._h, _fp(a), self.n_cells))

@@ -99,6 +99,8 @@ def lib() -> C.CDLL:
         "dsmc_get_ntc_state": ([P, pF, U32], I32),
         "dsmc_get_rng_state": ([P, pU, U32], I32),
         "dsmc_set_rng_state": ([P, pU, U32], I32),
+        "dsmc_get_taus_state": ([P, pU, U32], I32),
+        "dsmc_set_taus_state": ([P, pU, U32], I32),
         "dsmc_get_fluid_fraction": ([P, pF, U32], I32),
         "dsmc_get_collision_grid": ([P, pU, U32], I32),
         "dsmc_get_timings": ([P, pF, U32], I32),

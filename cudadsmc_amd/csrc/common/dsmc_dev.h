@@ -72,6 +72,7 @@ struct InstDesc {
   float vm_wall;    // sqrt(2 k T_wall / m); <= 0 specular walls (PHYSICS)
   float wall_temp;
   uint32_t mode;
+  uint32_t sample_on;     // this instance samples in launches flagged for sampling
   uint32_t periodic;      // REFERENCE box: 'x','y','z','n'
   uint32_t face_bc[6];    // PHYSICS box faces
   uint32_t has_geom;
@@ -116,6 +117,7 @@ struct InstDesc {
   double* wg_acc;       // [n_vwg][kAccSlots]
   uint32_t* hits;       // per sorted particle wall hits (debug, may be null)
   // ---- RNG
+  uint32_t* taus;       // REFERENCE with mt = 0: Hybrid-Taus state per cell, 4 words (else null)
   uint32_t* mt_state;   // [20][n_streams]
   const uint32_t* mt_a;
   const uint32_t* mt_b;
@@ -171,5 +173,36 @@ DM_HD void mt_seed(Mt& m, uint32_t seed) {
 #pragma unroll
   for (int i = 1; i < kMtN; ++i) m.st[i] = 1812433253u * (m.st[i - 1] ^ (m.st[i - 1] >> 30)) + (uint32_t)i;
 }
+
+// Generic draws for the NTC loop.
+DM_HD uint32_t rng_u32(Mt& m) { return mt_next(m); }
+DM_HD float rng_f(Mt& m) { return mt_float(m); }
+
+// ----------------------------------------------------------- Hybrid Taus
+// The reference's collision RNG when the Mersenne Twister is off
+// (HybridTausRng / HybridTausRngInt, kernels/mtwister.cu:118-153): three
+// Tausworthe components with the taus88 parameters of L'Ecuyer (1996),
+// (k, q, s) = (31, 13, 12), (29, 2, 4), (28, 3, 17), combined by XOR with the
+// LCG x <- 1664525 x + 1013904223 (mod 2^32). One state per cell.
+struct Taus {
+  uint32_t x, y, z, w;
+};
+
+DM_HD uint32_t taus_component(uint32_t& z, int s1, int s2, int s3, uint32_t m) {
+  uint32_t b = ((z << s1) ^ z) >> s2;
+  z = ((z & m) << s3) ^ b;
+  return z;
+}
+
+DM_HD uint32_t rng_u32(Taus& t) {
+  uint32_t a = taus_component(t.x, 13, 19, 12, 0xFFFFFFFEu);
+  uint32_t b = taus_component(t.y, 2, 25, 4, 0xFFFFFFF8u);
+  uint32_t c = taus_component(t.z, 3, 11, 17, 0xFFFFFFF0u);
+  t.w = t.w * 1664525u + 1013904223u;
+  return a ^ b ^ c ^ t.w;
+}
+
+// 2.3283064365387e-10 * u evaluated in double, rounded to float.
+DM_HD float rng_f(Taus& t) { return (float)(2.3283064365387e-10 * (double)rng_u32(t)); }
 
 }  // namespace dsmc_dev

@@ -412,13 +412,14 @@ __device__ __forceinline__ uint32_t move_physics(const InstDesc& I, float& px, f
 
 // ------------------------------------------------------------------- NTC
 // Isotropic post-collision velocities (dsmc_kernel.cu:1057-1078).
+template <class R>
 __device__ __forceinline__ void scatter_pair(float* vx, float* vy, float* vz, uint32_t i, uint32_t j, float mag,
-                                             Mt& rng) {
+                                             R& rng) {
   float v1x = vx[i], v1y = vy[i], v1z = vz[i];
   float v2x = vx[j], v2y = vy[j], v2z = vz[j];
-  float b = 2.0f * mt_float(rng) - 1.0f;
+  float b = 2.0f * rng_f(rng) - 1.0f;
   float a = sqrtf(1.0f - b * b);
-  float c = 6.28318548202514648438f * mt_float(rng);  // 2*DSMC_PI in float
+  float c = 6.28318548202514648438f * rng_f(rng);  // 2*DSMC_PI in float
   float sn, cs;
   dm_sincosf(c, &sn, &cs);
   float ax = b * mag / 2.0f, ay = a * mag * cs / 2.0f, az = a * mag * sn / 2.0f;
@@ -430,10 +431,12 @@ __device__ __forceinline__ void scatter_pair(float* vx, float* vy, float* vz, ui
   vz[j] = (v1z + v2z) / 2.0f - az;
 }
 
-// ComputeBirdCollisionsDev cell loop (dsmc_kernel.cu:1002-1089) on the n
+// ComputeBirdCollisionsDev / ComputeBirdCollisionsNoMTDev cell loop
+// (dsmc_kernel.cu:1002-1089 with MT, 912-1000 with Hybrid Taus) on the n
 // particles of one cell (velocity arrays already offset to the cell).
+template <class R>
 __device__ __forceinline__ void ntc_reference(const InstDesc& I, float* vx, float* vy, float* vz, uint32_t n,
-                                              float2& nt, float4& st, Mt& rng) {
+                                              float2& nt, float4& st, R& rng) {
   int part = (int)n;
   if (part < 2) return;
   float fpc = (float)((0.5 * (double)part * (double)I.avg_conc * (double)I.fnum * (double)nt.x *
@@ -446,12 +449,12 @@ __device__ __forceinline__ void ntc_reference(const InstDesc& I, float* vx, floa
     atomicOr(&I.ctr[3], kStatusPairs);
   }
   for (uint32_t k = 0; k < pairs; ++k) {
-    uint32_t i = mt_next(rng) % (uint32_t)part;
-    uint32_t j = mt_next(rng) % (uint32_t)part;
+    uint32_t i = rng_u32(rng) % (uint32_t)part;
+    uint32_t j = rng_u32(rng) % (uint32_t)part;
     float dx = vx[i] - vx[j], dy = vy[i] - vy[j], dz = vz[i] - vz[j];
     float mag = sqrtf(dx * dx + dy * dy + dz * dz);
     float v = mag * I.sigma_hs;
-    if (v / nt.x > mt_float(rng)) {
+    if (v / nt.x > rng_f(rng)) {
       st.y += 1.0f;
       st.z += mag;
       scatter_pair(vx, vy, vz, i, j, mag, rng);

@@ -452,7 +452,14 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
       float2 nt = I.ntc[c];
       if (MODE == kModeReference) {
         float4 st = I.cstat[c];
-        ntc_reference(I, vx + a, vy + a, vz + a, n, nt, st, rng);
+        if (I.taus) {
+          uint4 q = reinterpret_cast<const uint4*>(I.taus)[c];
+          Taus t = {q.x, q.y, q.z, q.w};
+          ntc_reference(I, vx + a, vy + a, vz + a, n, nt, st, t);
+          reinterpret_cast<uint4*>(I.taus)[c] = make_uint4(t.x, t.y, t.z, t.w);
+        } else {
+          ntc_reference(I, vx + a, vy + a, vz + a, n, nt, st, rng);
+        }
         I.cstat[c] = st;
       } else {
         float vol = I.cell_volume * (I.fluid_frac ? fmaxf(I.fluid_frac[c], kMinFluidFrac) : 1.0f);
@@ -497,7 +504,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
     count_keys(I.count, key);
   }
   // (6) sampling of the post-move velocities, one lane per cell
-  if (flags & kFlagSample) {
+  if ((flags & kFlagSample) && I.sample_on) {
     __syncthreads();
     if (tid < nc) {
       uint32_t c = c0 + tid;

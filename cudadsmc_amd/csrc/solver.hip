@@ -220,6 +220,7 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
     I.dim[a] = (uint32_t)s.grid_dim[a];
   }
   I.ncells = C;
+  I.sample_on = 1u;
   I.ntiles = (C + kTileCells - 1) / kTileCells;
   I.cell_volume = s.cell_size[0] * s.cell_size[1] * s.cell_size[2];
   I.mass = s.molmass;
@@ -363,6 +364,24 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
   std::vector<float2> nt(C);
   for (uint32_t c = 0; c < C; ++c) nt[c] = make_float2(h->ntc_init, phys ? 0.5f : 2.3283064365386963e-10f);
   HIPC(hipMemcpyAsync(I.ntc, nt.data(), (size_t)C * sizeof(float2), hipMemcpyHostToDevice, st));
+  if (!phys && !s.mt) {
+    // Hybrid-Taus collision streams, one per cell (ComputeBirdCollisionsNoMTDev).
+    // CreateRandomSeedsUint4 (kernels/dsmc_kernel.cu:1846-1886) reads
+    // /dev/urandom; here each word comes from splitmix64 of (seed, cell, k),
+    // reduced the same way (x mod (2^32 - 129) + 128), so runs repeat.
+    // dsmc_set_taus_state installs any other state.
+    CHECK_RC(h->alloc(&I.taus, (size_t)4 * C));
+    std::vector<uint32_t> ts((size_t)4 * C);
+    for (uint32_t c = 0; c < C; ++c)
+      for (uint32_t k = 0; k < 4; ++k) {
+        uint64_t z = ((uint64_t)s.seed << 40) + (uint64_t)c * 4u + k + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        ts[(size_t)4 * c + k] = (uint32_t)((z & 0xFFFFFFFFull) % 4294967167ull) + 128u;
+      }
+    HIPC(hipMemcpyAsync(I.taus, ts.data(), ts.size() * 4, hipMemcpyHostToDevice, st));
+  }
   HIPC(hipStreamSynchronize(st));
   h->scatter_grid = std::min<uint32_t>((cap + 255) / 256, 8192u);
   CHECK_RC(h->upload_desc());
@@ -964,6 +983,26 @@ int dsmc_set_rng_state(dsmc_solver* h, const uint32_t* in, uint32_t n) {
   return DSMC_OK;
 }
 
+int dsmc_get_taus_state(dsmc_solver* h, uint32_t* out, uint32_t n) {
+  if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  InstDesc& I = h->h;
+  if (!I.taus) return h->fail(DSMC_ERR_STATE, "no Hybrid-Taus state (REFERENCE mode with mt = 0 only)");
+  if (n < 4u * I.ncells) return h->fail(DSMC_ERR_INVALID, "need %u words", 4u * I.ncells);
+  HIPC(hipMemcpyAsync(out, I.taus, (size_t)16 * I.ncells, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  return DSMC_OK;
+}
+
+int dsmc_set_taus_state(dsmc_solver* h, const uint32_t* in, uint32_t n) {
+  if (!h || !in) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  InstDesc& I = h->h;
+  if (!I.taus) return h->fail(DSMC_ERR_STATE, "no Hybrid-Taus state (REFERENCE mode with mt = 0 only)");
+  if (n < 4u * I.ncells) return h->fail(DSMC_ERR_INVALID, "need %u words", 4u * I.ncells);
+  HIPC(hipMemcpyAsync(I.taus, in, (size_t)16 * I.ncells, hipMemcpyHostToDevice, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  return DSMC_OK;
+}
+
 int dsmc_get_fluid_fraction(dsmc_solver* h, float* out, uint32_t n) {
   if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
   InstDesc& I = h->h;
@@ -1286,6 +1325,8 @@ int dsmc_batch_create(dsmc_solver* const* hs, uint32_t n, dsmc_batch** out) {
     if (hs[i]->h.mode != hs[0]->h.mode) return set_global_error(DSMC_ERR_INVALID, "batch members differ in mode");
     if (hs[i]->par != hs[0]->par || hs[i]->steps != hs[0]->steps)
       return set_global_error(DSMC_ERR_STATE, "batch members must be at the same step");
+    if (!hs[i]->physics() && hs[i]->s.sample_every != hs[0]->s.sample_every)
+      return set_global_error(DSMC_ERR_INVALID, "REFERENCE batch members must share sample_every");
     for (uint32_t j = 0; j < i; ++j)
       if (hs[j] == hs[i]) return set_global_error(DSMC_ERR_INVALID, "a solver appears twice in the batch");
   }
@@ -1341,18 +1382,34 @@ int dsmc_batch_run(dsmc_batch* b, uint32_t steps) {
     BHIP(hipEventRecord(b->ev, h->stream));
     BHIP(hipStreamWaitEvent(b->stream, b->ev, 0));
   }
+  // Sampling: in PHYSICS mode each member samples while its own sampling is
+  // on (sample_on in its descriptor); launches carry the sample flag when any
+  // member samples. REFERENCE members share the every-Nth-step schedule.
   std::vector<InstDesc> descs(b->n);
-  for (uint32_t i = 0; i < b->n; ++i) descs[i] = b->hs[i]->h;
+  bool any_sampling = false;
+  for (uint32_t i = 0; i < b->n; ++i) {
+    descs[i] = b->hs[i]->h;
+    descs[i].sample_on = (!b->hs[i]->physics() || b->hs[i]->sampling) ? 1u : 0u;
+    any_sampling = any_sampling || b->hs[i]->sampling;
+  }
   BHIP(hipMemcpyAsync(b->d_descs, descs.data(), sizeof(InstDesc) * b->n, hipMemcpyHostToDevice, b->stream));
+  auto flags_at = [&](uint64_t stepno, bool* smp) {
+    uint32_t f = step_flags(h0, stepno, smp);
+    if (h0->physics()) {
+      *smp = any_sampling;
+      f = any_sampling ? (f | dsmc_launch::kFlagSample) : (f & ~dsmc_launch::kFlagSample);
+    }
+    return f;
+  };
   uint32_t done = 0;
   uint32_t par = h0->par;
   uint64_t stepno = h0->steps;
   uint32_t samples = 0;
   while (done < steps) {
     bool s1 = false, s2 = false;
-    uint32_t f1 = step_flags(h0, stepno + 1, &s1);
+    uint32_t f1 = flags_at(stepno + 1, &s1);
     if (b->use_graphs && steps - done >= 2) {
-      uint32_t f2 = step_flags(h0, stepno + 2, &s2);
+      uint32_t f2 = flags_at(stepno + 2, &s2);
       uint64_t key = ((uint64_t)par << 63) | ((uint64_t)f1 << 32) | f2;
       auto it = b->graphs.find(key);
       hipGraphExec_t exec;
@@ -1385,7 +1442,7 @@ int dsmc_batch_run(dsmc_batch* b, uint32_t steps) {
   for (auto* h : b->hs) {
     h->par = par;
     h->steps = stepno;
-    h->samples += samples;
+    if (!h->physics() || h->sampling) h->samples += samples;
   }
   // members' later calls run after the batch's work
   BHIP(hipEventRecord(b->ev, b->stream));

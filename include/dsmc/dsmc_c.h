@@ -200,6 +200,11 @@ int dsmc_get_rng_params(dsmc_solver* h, uint32_t* a, uint32_t* b, uint32_t* c, u
  * oldest first, and a zero), stream-major. */
 int dsmc_get_rng_state(dsmc_solver* h, uint32_t* out, uint32_t n_words);
 int dsmc_set_rng_state(dsmc_solver* h, const uint32_t* in, uint32_t n_words);
+/* Hybrid-Taus collision RNG state, 4 words {x, y, z, w} per cell (REFERENCE
+ * mode with mt = 0: ComputeBirdCollisionsNoMTDev, kernels/dsmc_kernel.cu:
+ * 912-1000, seeded per cell as CreateRandomSeedsUint4 does at :1846-1886). */
+int dsmc_get_taus_state(dsmc_solver* h, uint32_t* out, uint32_t n_words);
+int dsmc_set_taus_state(dsmc_solver* h, const uint32_t* in, uint32_t n_words);
 /* Per-cell fluid volume fraction used by PHYSICS mode (1 = no body). */
 int dsmc_get_fluid_fraction(dsmc_solver* h, float* out, uint32_t n_cells);
 /* Collision grid of the geometry: flags (0 empty, 1 triangles, 2 filled

@@ -67,6 +67,10 @@ def lib() -> C.CDLL:
         "oracle_get_cstat": ([P, pF, U32], I32),
         "oracle_get_acc": ([P, pD, U32], I32),
         "oracle_threads": ([], I32),
+        "oracle_set_taus": ([P, pU, U32], I32),
+        "oracle_get_taus": ([P, pU, U32], I32),
+        "oracle_taus_draws": ([pU, I32, U32, pU], None),
+        "oracle_taus_period": ([I32, U32, C.c_uint64], C.c_uint64),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -136,6 +140,19 @@ def stable_sort(keys, n_cells) -> tuple[np.ndarray, np.ndarray]:
     return perm[: int(cs[-1])], cs
 
 
+def taus_draws(seed, n: int, component: int = 0) -> np.ndarray:
+    """Hybrid-Taus outputs from state seed (x, y, z, w); component 1-3 = one
+    Tausworthe component, 4 = the LCG, 0 = the combined generator."""
+    sd = np.ascontiguousarray(seed, np.uint32)
+    out = np.zeros(n, np.uint32)
+    lib().oracle_taus_draws(up(sd), component, n, up(out))
+    return out
+
+
+def taus_period(component: int, v: int, limit: int) -> int:
+    return int(lib().oracle_taus_period(component, v, limit))
+
+
 def intersect(o, d, v0, v1, v2):
     arrs = [np.ascontiguousarray(x, np.float32) for x in (o, d, v0, v1, v2)]
     out = np.zeros(3, np.float32)
@@ -203,6 +220,15 @@ class OracleSim:
         w = np.zeros(20 * self.S, np.uint32)
         assert lib().oracle_get_rng(self._o, up(w), len(w)) == 0
         return w.reshape(self.S, 20)
+
+    def set_taus(self, words: np.ndarray) -> None:
+        w = np.ascontiguousarray(words, np.uint32).ravel()
+        assert lib().oracle_set_taus(self._o, up(w), len(w)) == 0
+
+    def taus(self) -> np.ndarray:
+        w = np.zeros(4 * self.C, np.uint32)
+        assert lib().oracle_get_taus(self._o, up(w), len(w)) == 0
+        return w.reshape(self.C, 4)
 
     def set_ntc(self, ntc: np.ndarray) -> None:
         a = np.ascontiguousarray(ntc, np.float32).ravel()

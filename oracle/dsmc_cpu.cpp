@@ -46,6 +46,26 @@ const float PI_REF = 3.14159265358979f;
 // candidate pairs per cell and step beyond which the engine clamps and flags
 const uint32_t kMaxPairs = 1u << 20;
 
+// ----------------------------------------------------------- Hybrid Taus
+// HybridTausRng / HybridTausRngInt / TausStep / LCGStep
+// (kernels/mtwister.cu:118-153): the collision RNG of
+// ComputeBirdCollisionsNoMTDev (kernels/dsmc_kernel.cu:912-1000).
+struct TausState {
+  uint32_t x = 0, y = 0, z = 0, w = 0;
+  static uint32_t step(uint32_t& v, int s1, int s2, int s3, unsigned long long mask) {
+    uint32_t b = ((v << s1) ^ v) >> s2;
+    v = (uint32_t)((((unsigned long long)v & mask) << s3) ^ b);
+    return v;
+  }
+  uint32_t next() {
+    uint32_t r = step(x, 13, 19, 12, 4294967294ULL) ^ step(y, 2, 25, 4, 4294967288ULL) ^
+                 step(z, 3, 11, 17, 4294967280ULL);
+    w = (uint32_t)((unsigned long long)w * 1664525ULL + 1013904223ULL);
+    return r ^ w;
+  }
+  float uniform() { return (float)(2.3283064365387e-10 * (double)next()); }
+};
+
 // ------------------------------------------------------------------ MT
 struct MtBlock {  // genrand_mt-style generator (own implementation)
   uint32_t n = 19, m = 9, a = 0, lower = 1, u = 12, s = 7, b = 0, t = 15, c = 0, l = 18;
@@ -144,6 +164,7 @@ struct oracle_sim {
   std::vector<float> frac;
   // RNG streams
   std::vector<MtBlock> rng;
+  std::vector<TausState> taus;  // per cell, REFERENCE mode with mt = 0 (empty otherwise)
   // state in storage order
   std::vector<float> x, y, z, vx, vy, vz;
   std::vector<uint32_t> key;
@@ -410,7 +431,8 @@ struct oracle_sim {
   }
 
   // ------------------------------------------------------------- NTC
-  static void scatter(float* ux, float* uy, float* uz, uint32_t i, uint32_t j, float mag, MtBlock& r) {
+  template <class R>
+  static void scatter(float* ux, float* uy, float* uz, uint32_t i, uint32_t j, float mag, R& r) {
     float v1x = ux[i], v1y = uy[i], v1z = uz[i], v2x = ux[j], v2y = uy[j], v2z = uz[j];
     float b = 2.0f * r.uniform() - 1.0f;
     float a = std::sqrt(1 - b * b);
@@ -426,7 +448,8 @@ struct oracle_sim {
     uz[j] = (v1z + v2z) / 2.0f - az;
   }
 
-  void ntc_reference(float* ux, float* uy, float* uz, int part, float* nt, float* st, MtBlock& r) const {
+  template <class R>
+  void ntc_reference(float* ux, float* uy, float* uz, int part, float* nt, float* st, R& r) const {
     if (part < 2) return;
     float fpc = (0.5 * part * avg_conc * fnum * nt[0] * dt) / cell_volume + nt[1];
     uint32_t pairs = dm_f2u_sat(fpc);
@@ -554,6 +577,9 @@ struct oracle_sim {
           if (phys) {
             float vol = cell_volume * (frac.empty() ? 1.0f : std::max(frac[c], 1.0f / 64.0f));
             ntc_physics(&nvx[a0], &nvy[a0], &nvz[a0], b0 - a0, vol, &ntc[2 * (size_t)c], r, a);
+          } else if (!taus.empty()) {
+            ntc_reference(&nvx[a0], &nvy[a0], &nvz[a0], (int)(b0 - a0), &ntc[2 * (size_t)c], &cstat[4 * (size_t)c],
+                          taus[c]);
           } else {
             ntc_reference(&nvx[a0], &nvy[a0], &nvz[a0], (int)(b0 - a0), &ntc[2 * (size_t)c], &cstat[4 * (size_t)c], r);
           }
@@ -927,6 +953,56 @@ int oracle_get_rng(oracle_sim* o, uint32_t* w, uint32_t n) {
   for (uint32_t i = 0; i < o->S; ++i) {
     o->rng[i].to_window(&w[(size_t)i * 20]);
     w[(size_t)i * 20 + 19] = 0u;
+  }
+  return 0;
+}
+
+int oracle_set_taus(oracle_sim* o, const uint32_t* w, uint32_t n) {
+  if (!o || n < 4u * o->C) return -1;
+  o->taus.resize(o->C);
+  for (uint32_t c = 0; c < o->C; ++c) o->taus[c] = {w[4 * c], w[4 * c + 1], w[4 * c + 2], w[4 * c + 3]};
+  return 0;
+}
+
+int oracle_get_taus(oracle_sim* o, uint32_t* w, uint32_t n) {
+  if (!o || o->taus.empty() || n < 4u * o->C) return -1;
+  for (uint32_t c = 0; c < o->C; ++c) {
+    w[4 * c] = o->taus[c].x;
+    w[4 * c + 1] = o->taus[c].y;
+    w[4 * c + 2] = o->taus[c].z;
+    w[4 * c + 3] = o->taus[c].w;
+  }
+  return 0;
+}
+
+// Draws of one Hybrid-Taus stream (the combined generator, or with
+// component = 1..3 one Tausworthe component alone, 4 the LCG alone).
+void oracle_taus_draws(const uint32_t seed[4], int component, uint32_t n, uint32_t* out) {
+  TausState t{seed[0], seed[1], seed[2], seed[3]};
+  for (uint32_t i = 0; i < n; ++i) {
+    switch (component) {
+      case 1: out[i] = TausState::step(t.x, 13, 19, 12, 4294967294ULL); break;
+      case 2: out[i] = TausState::step(t.y, 2, 25, 4, 4294967288ULL); break;
+      case 3: out[i] = TausState::step(t.z, 3, 11, 17, 4294967280ULL); break;
+      case 4: t.w = (uint32_t)((unsigned long long)t.w * 1664525ULL + 1013904223ULL); out[i] = t.w; break;
+      default: out[i] = t.next();
+    }
+  }
+}
+
+// Period of one Tausworthe component from state v (steps until the state
+// returns; 0 if it has not after `limit` steps).
+uint64_t oracle_taus_period(int component, uint32_t v, uint64_t limit) {
+  static const int sh[3][3] = {{13, 19, 12}, {2, 25, 4}, {3, 11, 17}};
+  static const unsigned long long mk[3] = {4294967294ULL, 4294967288ULL, 4294967280ULL};
+  if (component < 1 || component > 3) return 0;
+  const int* q = sh[component - 1];
+  uint32_t z = v;
+  TausState::step(z, q[0], q[1], q[2], mk[component - 1]);  // enter the cycle (low bits masked off)
+  const uint32_t z0 = z;
+  for (uint64_t k = 1; k <= limit; ++k) {
+    TausState::step(z, q[0], q[1], q[2], mk[component - 1]);
+    if (z == z0) return k;
   }
   return 0;
 }

@@ -65,3 +65,35 @@ def test_batch_rejects_mismatched_members():
         dm.Batch([a, b])
     with pytest.raises(dm.DsmcError, match="twice"):
         dm.Batch([b, b])
+
+
+def test_batch_members_sample_independently():
+    """Only the members whose sampling is on accumulate cell samples."""
+    print("batch per-member sampling", flush=True)
+    cases = CASES[:2]
+    alone = []
+    for i, (seed, n, U, r) in enumerate(cases):
+        sol = make(seed, n, U, r)
+        sol.set_graphs(False)
+        sol.DoBenchmark(4, sol.settings.dt)
+        if i == 0:
+            sol.set_sampling(True)
+        sol.DoBenchmark(6, sol.settings.dt)
+        alone.append((sol.particles(), sol.cell_stats(), sol.counts()[2], sol.drag()))
+        sol.close()
+    members = [make(seed, n, U, r) for seed, n, U, r in cases]
+    batch = dm.Batch(members)
+    batch.run(4)
+    members[0].set_sampling(True)
+    batch.run(6)
+    batch.synchronize()
+    for sol, (p, cs, ns, drag) in zip(members, alone):
+        q = sol.particles()
+        for k in ("pos", "vel", "key"):
+            assert np.array_equal(p[k], q[k]), k
+        assert np.array_equal(cs, sol.cell_stats())
+        assert sol.counts()[2] == ns
+        assert np.allclose(drag, sol.drag(), rtol=1e-12, atol=0)
+    assert alone[0][2] == 6 and alone[1][2] == 0
+    assert alone[1][1].sum() == 0.0 and alone[0][1].sum() > 0.0
+    batch.close()

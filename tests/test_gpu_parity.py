@@ -154,11 +154,15 @@ def _mirror(sol: dm.Solver, mesh=None) -> oracle.OracleSim:
     return o
 
 
-@pytest.mark.parametrize("periodic,geom", [("x", True), ("z", True), ("n", False), ("y", True)])
-def test_gpu_reference_steps_bit_exact(periodic, geom):
-    note(f"reference step parity periodic={periodic} geometry={geom}")
+@pytest.mark.parametrize("periodic,geom,mt", [("x", True, 1), ("z", True, 1), ("n", False, 1), ("y", True, 1),
+                                               ("x", True, 0), ("n", False, 0)])
+def test_gpu_reference_steps_bit_exact(periodic, geom, mt):
+    """mt = 1: MT collision streams; mt = 0: the reference's default Hybrid-Taus
+    per-cell streams (ComputeBirdCollisionsNoMTDev)."""
+    note(f"reference step parity periodic={periodic} geometry={geom} mt={mt}")
     s = small_reference_settings(periodic=periodic, geometry=geom)
     s.sample_every = 3
+    s.mt = mt
     sol = dm.Solver()
     sol.Init(s)
     mesh = None
@@ -166,6 +170,8 @@ def test_gpu_reference_steps_bit_exact(periodic, geom):
         mesh = dm.icosphere((0.07, 0.06, 0.05), 0.022, 2)
         sol.SetModel(mesh, grid=(6, 6, 6))
     o = _mirror(sol, mesh)
+    if not mt:
+        o.set_taus(sol.taus_state())
     U = (900.0, 150.0, -120.0)
     n = sol.InitSystemState(U)
     o.load(U)
@@ -186,6 +192,8 @@ def test_gpu_reference_steps_bit_exact(periodic, geom):
         total_hits += int(hits.sum())
         _compare_state(sol, o, tag)
         assert np.array_equal(sol.ntc_state(), o.ntc()), f"{tag}: ntc"
+        if not mt:
+            assert np.array_equal(sol.taus_state(), o.taus()), f"{tag}: taus state"
     assert np.array_equal(sol.rng_state(), o.rng()), "rng after steps"
     cs = sol.cell_stats()
     assert np.array_equal(cs, o.samples()), "samples"

@@ -116,3 +116,32 @@ def test_stream_statistics():
     cc = np.corrcoef(x.T)
     off = cc[~np.eye(ns, dtype=bool)]
     assert np.abs(off).max() < 6 / np.sqrt(4000)
+
+
+# ------------------------------------------------------------- Hybrid Taus
+def test_taus_lcg_component_closed_form():
+    """x_n = a^n x_0 + c (a^n - 1) / (a - 1) mod 2^32 (geometric series)."""
+    a, c, x0 = 1664525, 1013904223, 2718281828
+    got = oracle.taus_draws([200, 300, 400, x0], 1000, component=4)
+    M = 2 ** 32
+    for n in (1, 2, 3, 17, 999, 1000):
+        an = pow(a, n, M * (a - 1))
+        want = (an * x0 + c * ((an - 1) // (a - 1))) % M
+        assert int(got[n - 1]) == want
+
+
+@pytest.mark.parametrize("component,k", [(3, 28), (2, 29), (1, 31)])
+def test_taus_component_has_maximal_period(component, k):
+    """taus88 (L'Ecuyer 1996): each component is a maximal-period Tausworthe
+    generator, period 2^k - 1 on its k significant bits; a wrong shift or mask
+    makes the characteristic polynomial non-primitive and the period shorter."""
+    for v in (128, 0xDEADBEEF):
+        assert oracle.taus_period(component, v, 2 ** k) == 2 ** k - 1
+
+
+def test_taus_combination_and_float():
+    seed = [12345, 67890, 13579, 24680]
+    comb = oracle.taus_draws(seed, 500)
+    parts = [oracle.taus_draws(seed, 500, component=c) for c in (1, 2, 3, 4)]
+    np.testing.assert_array_equal(comb, parts[0] ^ parts[1] ^ parts[2] ^ parts[3])
+    assert len(set(comb.tolist())) == 500
