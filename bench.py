@@ -169,6 +169,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized buffer 5 times (calibrates HBM byte counters)")
     ap.add_argument("--ensemble-steps", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -176,9 +178,23 @@ def main() -> int:
               "design-only in this build", file=sys.stderr)
         return 2
 
+    if args.copy_probe:
+        import torch
+
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n, "copies": 5}))
+        return 0
+
     import cudadsmc_amd as dm
+    from cudadsmc_amd import ensemble as ens
 
     s, mesh, cgrid = make_case(args.workload)
+    s.n_streams = int(ens.tuned(args.workload, 262144))
     t_setup = time.perf_counter()
     sol = dm.Solver()
     sol.Init(s)

@@ -260,6 +260,10 @@ class Solver:
         w = np.ascontiguousarray(words, np.uint32).ravel()
         self._c(lib().dsmc_set_rng_state(self._h, _up(w), len(w)))
 
+    def set_tile_cells(self, n: int) -> None:
+        """Cells per tile of the tile kernel (power of two in [32, 256]), before loading."""
+        self._c(lib().dsmc_set_tile_cells(self._h, int(n)))
+
     def taus_state(self) -> np.ndarray:
         """Hybrid-Taus collision RNG state (REFERENCE mode with mt = 0), (cells, 4)."""
         return self._u32(lib().dsmc_get_taus_state, 4 * self.n_cells).reshape(self.n_cells, 4)

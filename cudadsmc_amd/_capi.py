@@ -99,6 +99,7 @@ def lib() -> C.CDLL:
         "dsmc_get_ntc_state": ([P, pF, U32], I32),
         "dsmc_get_rng_state": ([P, pU, U32], I32),
         "dsmc_set_rng_state": ([P, pU, U32], I32),
+        "dsmc_set_tile_cells": ([P, U32], I32),
         "dsmc_get_taus_state": ([P, pU, U32], I32),
         "dsmc_set_taus_state": ([P, pU, U32], I32),
         "dsmc_get_fluid_fraction": ([P, pF, U32], I32),

@@ -100,3 +100,51 @@ def _load(path) -> dict:
         except json.JSONDecodeError:
             return {}
     return {}
+
+
+def main(argv=None) -> None:
+    """Tune the ensemble (64 instances of the 64x32x32 sphere case) and the
+    large single case; write cudadsmc_amd/tuned.json."""
+    import argparse
+
+    from . import ensemble as ens
+
+    ap = argparse.ArgumentParser(description="tune launch parameters on the GPU")
+    ap.add_argument("--what", default="ensemble64,c3", help="comma list of ensemble64, c3")
+    ap.add_argument("--out", default=str(ens.TUNED))
+    a = ap.parse_args(argv)
+    out = Path(a.out)
+    db = _load(out)
+    for what in a.what.split(","):
+        if what == "ensemble64":
+            cases = ens.draw_cases(64, seed=2024, temperatures=False)
+            res = tune_batch(lambda ns: [ens.make_solver(c, n_streams=ns) for c in cases],
+                             streams=(8192, 16384, 32768, 65536), warmup=30, steps=60)
+        elif what == "c3":
+            import sys
+
+            sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+            from bench import make_case  # the benchmark's workload definition
+
+            from . import Solver
+
+            def make(ns):
+                s, mesh, cg = make_case("c3")
+                s.n_streams = ns
+                sol = Solver()
+                sol.Init(s)
+                sol.SetModel(mesh, grid=cg)
+                sol.InitSystemState(tuple(s.stream_velocity))
+                return sol
+
+            res = tune_solver(make, streams=(131072, 196608, 262144, 393216, 524288), scatter=(4096, 8192, 16384),
+                              warmup=20, steps=40)
+        else:
+            raise SystemExit(f"unknown tuning target {what}")
+        db[what] = res
+        print(what, json.dumps({k: v for k, v in res.items() if k != "table"}), flush=True)
+        out.write_text(json.dumps(db, indent=1))
+
+
+if __name__ == "__main__":
+    main()

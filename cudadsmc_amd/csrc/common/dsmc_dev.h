@@ -19,7 +19,7 @@
 namespace dsmc_dev {
 
 constexpr uint32_t kInvalidKey = 0xFFFFFFFFu;
-constexpr uint32_t kTileCells = 256;    // cells per tile (one lane per cell)
+constexpr uint32_t kTileCells = 256;    // most cells per tile (one lane per cell)
 constexpr uint32_t kTileThreads = 256;  // threads of the tile kernel
 constexpr uint32_t kLdsCap = 2048;      // particles staged in LDS per tile
 constexpr uint32_t kMtWords = 20;       // 19 window words + 1 spare (zero)
@@ -36,6 +36,7 @@ constexpr float kGeomEps = 0.0001f;               // EPS, dsmc_kernel.cu:1251
 constexpr float kModelWidth = 0.0001f;            // MODEL_WIDTH, dsmc_kernel.cu:1253
 constexpr int kCollisionLimit = 10;               // COLLISION_LIMIT, dsmc_kernel.cu:1250
 constexpr double kTriEpsilon = 0.000001;          // EPSILON, tri_intersect.cu:1
+constexpr double kTriRelEpsilon = 1.0e-9;         // PHYSICS: |det| vs (|e1|^2 + |e2|^2)
 constexpr int kPhysMaxBounces = 8;
 // Smallest fluid fraction used for a cell's NTC volume: the fraction is
 // sampled at 4^3 points, so a cell reported empty may still hold a sliver of
@@ -59,6 +60,7 @@ struct InstDesc {
   float gmin[3], gmax[3], cell[3];
   uint32_t dim[3];
   uint32_t ncells;
+  uint32_t tile_cells;    // cells per tile: a power of two in [32, kTileCells]
   uint32_t ntiles;
   float cell_volume;
   // ---- gas and NTC

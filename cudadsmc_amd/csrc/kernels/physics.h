@@ -48,6 +48,10 @@ __device__ __forceinline__ uint32_t cg_cell(const InstDesc& I, float x, float y,
 // ----------------------------------------------------------- Moller-Trumbore
 // intersect_triangle (tri_intersect.cu:14-56): non-culling, EPSILON and
 // inv_det in double, the rest in float.
+// The reference's parallel test |det| < EPSILON is absolute (m^2), so it
+// rejects every hit on triangles of a few mm at oblique incidence; PHYSICS
+// mode (REL = true) scales the threshold with the triangle instead.
+template <bool REL = false>
 __device__ __forceinline__ bool intersect_tri(float ox, float oy, float oz, float dx, float dy, float dz,
                                               float4 a, float4 b, float4 c, float& t, float& u, float& v) {
   float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
@@ -56,7 +60,10 @@ __device__ __forceinline__ bool intersect_tri(float ox, float oy, float oz, floa
   float py = dz * e2x - dx * e2z;
   float pz = dx * e2y - dy * e2x;
   float det = e1x * px + e1y * py + e1z * pz;
-  if ((double)det > -kTriEpsilon && (double)det < kTriEpsilon) return false;
+  const double eps = REL ? kTriRelEpsilon * ((double)(e1x * e1x + e1y * e1y + e1z * e1z) +
+                                            (double)(e2x * e2x + e2y * e2y + e2z * e2z))
+                         : kTriEpsilon;
+  if ((double)det > -eps && (double)det < eps) return false;
   float inv_det = (float)(1.0 / (double)det);
   float tx = ox - a.x, ty = oy - a.y, tz = oz - a.z;
   u = (tx * px + ty * py + tz * pz) * inv_det;
@@ -272,7 +279,7 @@ __device__ __forceinline__ bool phys_nearest_hit(const InstDesc& I, float ox, fl
         float4 n = I.tn[tri];
         if (!(dx * n.x + dy * n.y + dz * n.z < 0.0f)) continue;
         float t, u, v;
-        if (intersect_tri(ox, oy, oz, dx, dy, dz, I.tv0[tri], I.tv1[tri], I.tv2[tri], t, u, v)) {
+        if (intersect_tri<true>(ox, oy, oz, dx, dy, dz, I.tv0[tri], I.tv1[tri], I.tv2[tri], t, u, v)) {
           if (t >= 0.0f && t <= dist) {
             if (!found || t < best || (t == best && tri < best_tri)) {
               found = true;

@@ -568,8 +568,8 @@ __global__ void __launch_bounds__(kTileThreads) k_tile(const InstDesc* __restric
   if (vw == 0 && tid == 0) I.ctr[0] = I.ctr[1];
 
   for (uint32_t tile = vw; tile < I.ntiles; tile += I.n_vwg) {
-    const uint32_t c0 = tile * kTileCells;
-    const uint32_t nc = min(kTileCells, I.ncells - c0);
+    const uint32_t c0 = tile * I.tile_cells;
+    const uint32_t nc = min(I.tile_cells, I.ncells - c0);
     s_start[tid] = I.cell_start[c0 + min(tid, nc)];
     if (tid == 0) s_start[nc] = I.cell_start[c0 + nc];
     __syncthreads();

@@ -197,6 +197,9 @@ void dsmc_destroy(dsmc_solver* h) { delete h; }
 
 const char* dsmc_last_error(const dsmc_solver* h) { return h ? h->err.c_str() : global_error(); }
 
+static int physics_fnum(dsmc_solver* h);
+static int setup_injection(dsmc_solver* h);
+
 int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
   if (!h || !sp) return set_global_error(DSMC_ERR_INVALID, "null argument");
   if (h->inited) return h->fail(DSMC_ERR_STATE, "solver already initialised");
@@ -221,6 +224,7 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
   }
   I.ncells = C;
   I.sample_on = 1u;
+  I.tile_cells = kTileCells;
   I.ntiles = (C + kTileCells - 1) / kTileCells;
   I.cell_volume = s.cell_size[0] * s.cell_size[1] * s.cell_size[2];
   I.mass = s.molmass;
@@ -276,6 +280,17 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
   }
   uint32_t ns = s.n_streams ? round_up(s.n_streams, kTileThreads) : 262144u;
   if (!phys) ns = std::max(ns, round_up(std::max(C, (uint32_t)(s.grid_dim[0] * s.grid_dim[1]) * h->ppc), kTileThreads));
+  // Cells per tile: keep the expected particles of a tile near half the LDS
+  // capacity so that few tiles take the global-memory path.
+  {
+    double ppc_exp = phys ? (s.target_particles ? (double)s.target_particles / C : (double)s.density *
+                             I.cell_volume / (double)s.fnum)
+                          : (double)h->ppc;
+    uint32_t tc = kTileCells;
+    while (tc > 32u && tc * ppc_exp > 0.5 * kLdsCap) tc >>= 1;
+    I.tile_cells = tc;
+    I.ntiles = (C + tc - 1) / tc;
+  }
   I.cap = cap;
   I.n_streams = ns;
   I.n_vwg = ns / kTileThreads;
@@ -384,6 +399,12 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
   }
   HIPC(hipStreamSynchronize(st));
   h->scatter_grid = std::min<uint32_t>((cap + 255) / 256, 8192u);
+  if (phys) {
+    // F_n and the inflow per face for the whole box; dsmc_set_geometry
+    // redoes both for the fluid volume
+    CHECK_RC(physics_fnum(h));
+    CHECK_RC(setup_injection(h));
+  }
   CHECK_RC(h->upload_desc());
   HIPC(hipStreamSynchronize(st));
   h->inited = true;
@@ -475,6 +496,9 @@ int dsmc_set_geometry(dsmc_solver* h, const float* verts, uint32_t nv, const uin
     CHECK_RC(h->alloc(&df, I.ncells, true));
     HIPC(hipMemcpyAsync(df, h->frac.data(), (size_t)I.ncells * 4, hipMemcpyHostToDevice, st));
     I.fluid_frac = df;
+    // F_n and the inflow per face depend on the fluid volume
+    CHECK_RC(physics_fnum(h));
+    CHECK_RC(setup_injection(h));
   }
   CHECK_RC(h->upload_desc());
   HIPC(hipStreamSynchronize(st));
@@ -516,7 +540,7 @@ static int setup_injection(dsmc_solver* h) {
     I.face_off[f + 1] = total;
   }
   I.n_face_cells = total;
-  if (total) CHECK_RC(h->alloc(&I.face_count, (size_t)total + 1));
+  if (total && !I.face_count) CHECK_RC(h->alloc(&I.face_count, (size_t)total + 1));
   return DSMC_OK;
 }
 
@@ -983,6 +1007,17 @@ int dsmc_set_rng_state(dsmc_solver* h, const uint32_t* in, uint32_t n) {
   return DSMC_OK;
 }
 
+int dsmc_set_tile_cells(dsmc_solver* h, uint32_t tile_cells) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->inited) return h->fail(DSMC_ERR_STATE, "not initialised");
+  if (h->have_state) return h->fail(DSMC_ERR_STATE, "tile size must be set before the particles are loaded");
+  if (tile_cells < 32u || tile_cells > kTileCells || (tile_cells & (tile_cells - 1u)))
+    return h->fail(DSMC_ERR_INVALID, "tile_cells must be a power of two in [32, %u]", kTileCells);
+  h->h.tile_cells = tile_cells;
+  h->h.ntiles = (h->h.ncells + tile_cells - 1) / tile_cells;
+  return h->upload_desc();
+}
+
 int dsmc_get_taus_state(dsmc_solver* h, uint32_t* out, uint32_t n) {
   if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
   InstDesc& I = h->h;
@@ -1079,7 +1114,7 @@ int dsmc_set_debug(dsmc_solver* h, uint32_t flags) {
 
 int dsmc_get_layout(dsmc_solver* h, uint32_t* tile_cells, uint32_t* threads, uint32_t* n_streams, uint32_t* n_vwg) {
   if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
-  if (tile_cells) *tile_cells = kTileCells;
+  if (tile_cells) *tile_cells = h->h.tile_cells;
   if (threads) *threads = kTileThreads;
   if (n_streams) *n_streams = h->h.n_streams;
   if (n_vwg) *n_vwg = h->h.n_vwg;

@@ -177,7 +177,8 @@ def run_batch(cases: list[Case], transient: int, sampling: int, windows: int = 2
             d.update(s, done)
         t_read += time.perf_counter() - tr
     t_sampling = time.perf_counter() - t2
-    out = {"cases": [asdict(c) for c in cases], "fields": [], "cd": [], "cd_sigma": [], "n_particles": [],
+    out = {"cases": [asdict(c) for c in cases], "fields": [], "counts": [], "cd": [], "cd_sigma": [],
+           "n_particles": [],
            "fluid_fraction": None, "timing": {"setup_s": t_setup, "transient_s": t_transient,
                                               "sampling_s": t_sampling, "readout_s": t_read,
                                               "steps": transient + sampling}}
@@ -185,6 +186,7 @@ def run_batch(cases: list[Case], transient: int, sampling: int, windows: int = 2
     for d, s in zip(drags, sols):
         f = s.fields()
         out["fields"].append(f)
+        out["counts"].append(s.cell_stats()[:, 0].astype(np.float32))
         m, e = d.cd()
         out["cd"].append(m)
         out["cd_sigma"].append(e)

@@ -11,7 +11,7 @@ Architecture (about 2.1M parameters at width 96):
 * conv 3x3x3 32 -> W, ReLU;
 * residual blocks x <- x + ReLU(conv 3x3x3 W -> W, dilation d_i);
 * 1x1 head W -> 32, of which the first five channels are the fields
-  (log n/n_inf, u/U, v/U, w/U, log T/T_inf).
+  (n/n_inf, u/U, v/U, w/U, T/T_inf).
 
 Training minimises the masked mean-squared error with Adam on FP32 master
 weights. Every tensor operation of the forward and backward passes is a
@@ -122,16 +122,16 @@ def normalise_params(U, n, T_inf, T_w, r) -> np.ndarray:
 
 
 def fields_to_targets(n, u, v, w, T, n_inf, U, T_inf) -> np.ndarray:
-    """Fields of one case ([Z][Y][X] each) -> the five operator outputs."""
-    eps = 1e-30
-    return np.stack([np.log(np.maximum(n, eps) / n_inf), u / U, v / U, w / U,
-                     np.log(np.maximum(T, 1e-3) / T_inf)], axis=-1).astype(np.float32)
+    """Fields of one case ([Z][Y][X] each) -> the five operator outputs,
+    each scaled by its freestream value."""
+    return np.stack([np.asarray(n) / n_inf, np.asarray(u) / U, np.asarray(v) / U, np.asarray(w) / U,
+                     np.asarray(T) / T_inf], axis=-1).astype(np.float32)
 
 
 def targets_to_fields(t, n_inf, U, T_inf):
     """Inverse of fields_to_targets: returns n, u, v, w, T."""
     t = np.asarray(t, np.float64)
-    return (n_inf * np.exp(t[..., 0]), U * t[..., 1], U * t[..., 2], U * t[..., 3], T_inf * np.exp(t[..., 4]))
+    return (n_inf * t[..., 0], U * t[..., 1], U * t[..., 2], U * t[..., 3], T_inf * t[..., 4])
 
 
 # ------------------------------------------------------------------ network

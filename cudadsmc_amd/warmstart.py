@@ -65,7 +65,9 @@ def generate(out: Path, runs: int, batch: int, transient: int, sampling: int, se
             tg[b0 + i] = neural.fields_to_targets(f[..., 0], f[..., 1], f[..., 2], f[..., 3], f[..., 4], c.n, c.U,
                                                   c.T_inf)
             ff = np.asarray(r["fluid_fraction"][i]).reshape(Z, Y, X)
-            mk[b0 + i] = ((ff > 0.5) & (f[..., 0] > 0)).astype(np.float32)
+            cnt = np.asarray(r["counts"][i]).reshape(Z, Y, X)
+            # fluid cells with enough sampled particles for a usable temperature
+            mk[b0 + i] = ((ff > 0.5) & (cnt >= 20)).astype(np.float32)
             cd[b0 + i] = r["cd"][i]
             cds[b0 + i] = r["cd_sigma"][i]
             np_total += r["n_particles"][i]
@@ -129,13 +131,14 @@ def predicted_fields(op: neural.FieldOperator, cases: list[ens.Case]) -> list[di
     for c, t in zip(cases, pred):
         n, u, v, w, T = neural.targets_to_fields(t, c.n, c.U, c.T_inf)
         ok = all(np.all(np.isfinite(a)) for a in (n, u, v, w, T))
-        ok = ok and float(np.max(n)) < 100.0 * c.n and float(np.min(T)) > 1.0 and float(np.max(T)) < 1e5
+        ok = ok and float(np.max(n)) < 100.0 * c.n and float(np.max(T)) < 100.0 * c.T_inf
         if not ok:
             out.append(None)
             continue
-        out.append({"n": n.astype(np.float32).ravel(), "u": u.astype(np.float32).ravel(),
+        # clip the few non-physical cells (negative density or temperature)
+        out.append({"n": np.maximum(n, 0.0).astype(np.float32).ravel(), "u": u.astype(np.float32).ravel(),
                     "v": v.astype(np.float32).ravel(), "w": w.astype(np.float32).ravel(),
-                    "T": np.maximum(T, 1.0).astype(np.float32).ravel()})
+                    "T": np.maximum(T, 0.05 * c.T_inf).astype(np.float32).ravel()})
     return out
 
 

@@ -200,6 +200,12 @@ int dsmc_get_rng_params(dsmc_solver* h, uint32_t* a, uint32_t* b, uint32_t* c, u
  * oldest first, and a zero), stream-major. */
 int dsmc_get_rng_state(dsmc_solver* h, uint32_t* out, uint32_t n_words);
 int dsmc_set_rng_state(dsmc_solver* h, const uint32_t* in, uint32_t n_words);
+/* Cells per tile of the cell-tile kernel (a power of two in [32, 256]),
+ * before loading. The default keeps a tile's expected particles near half
+ * the LDS staging capacity (2048). Changes the RNG assignment (a run is
+ * reproducible for a given value), not the physics. No reference
+ * counterpart: the reference launches one thread per cell. */
+int dsmc_set_tile_cells(dsmc_solver* h, uint32_t tile_cells);
 /* Hybrid-Taus collision RNG state, 4 words {x, y, z, w} per cell (REFERENCE
  * mode with mt = 0: ComputeBirdCollisionsNoMTDev, kernels/dsmc_kernel.cu:
  * 912-1000, seeded per cell as CreateRandomSeedsUint4 does at :1846-1886). */

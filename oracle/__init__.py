@@ -71,6 +71,7 @@ def lib() -> C.CDLL:
         "oracle_get_taus": ([P, pU, U32], I32),
         "oracle_taus_draws": ([pU, I32, U32, pU], None),
         "oracle_taus_period": ([I32, U32, C.c_uint64], C.c_uint64),
+        "oracle_mt_window_after": ([U32, U32, U32, U32, U32, pU], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -146,6 +147,13 @@ def taus_draws(seed, n: int, component: int = 0) -> np.ndarray:
     sd = np.ascontiguousarray(seed, np.uint32)
     out = np.zeros(n, np.uint32)
     lib().oracle_taus_draws(up(sd), component, n, up(out))
+    return out
+
+
+def mt_window_after(a: int, b: int, c: int, seed: int, d: int) -> np.ndarray:
+    """Sliding-window state (19 words + a zero) of one stream after d draws."""
+    out = np.zeros(20, np.uint32)
+    lib().oracle_mt_window_after(a, b, c, seed, d, up(out))
     return out
 
 

@@ -70,9 +70,11 @@ struct TausState {
 struct MtBlock {  // genrand_mt-style generator (own implementation)
   uint32_t n = 19, m = 9, a = 0, lower = 1, u = 12, s = 7, b = 0, t = 15, c = 0, l = 18;
   std::vector<uint32_t> st;
+  std::vector<uint32_t> prev;  // the block before the last regen (for to_window)
   uint32_t i = 0;
   void seed(uint32_t sd) {
     st.assign(n, 0u);
+    prev.clear();
     st[0] = sd;
     for (uint32_t k = 1; k < n; ++k) st[k] = 1812433253u * (st[k - 1] ^ (st[k - 1] >> 30)) + k;
     i = n;
@@ -82,6 +84,7 @@ struct MtBlock {  // genrand_mt-style generator (own implementation)
     return (y >> 1) ^ ((y & 1u) ? a : 0u);
   }
   void regen() {
+    prev = st;
     for (uint32_t k = 0; k < n; ++k) {
       uint32_t nxt = st[(k + 1) % n];
       st[k] = st[(k + m) % n] ^ tw(st[k], nxt);
@@ -99,11 +102,15 @@ struct MtBlock {  // genrand_mt-style generator (own implementation)
   }
   float uniform() { return ((float)next() + 1.0f) / 4294967296.0f; }
   // window of the last n words, oldest first (the engine's state layout)
+  // Sliding window x_d .. x_{d+18} after d draws: with i words of the
+  // current block consumed, the tail of the previous block then the head of
+  // the current one.
   void to_window(uint32_t* w) const {
-    for (uint32_t k = 0; k < n; ++k) w[k] = st[(i + k) % n];
+    for (uint32_t k = 0; k < n; ++k) w[k] = (i + k < n) ? prev[i + k] : st[i + k - n];
   }
   void from_window(const uint32_t* w) {
     st.assign(w, w + n);
+    prev.clear();
     i = n;
   }
 };
@@ -118,13 +125,16 @@ struct V3 {
 };
 
 bool mt_intersect(float ox, float oy, float oz, float dx, float dy, float dz, const V3& a, const V3& b, const V3& c,
-                  float& t, float& u, float& v) {
-  // intersect_triangle (tri_intersect.cu:14-56)
+                  float& t, float& u, float& v, bool rel = false) {
+  // intersect_triangle (tri_intersect.cu:14-56); rel: PHYSICS mode's
+  // parallel threshold 1e-9 (|e1|^2 + |e2|^2) instead of the absolute 1e-6
   float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
   float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
   float px = dy * e2z - dz * e2y, py = dz * e2x - dx * e2z, pz = dx * e2y - dy * e2x;
   float det = e1x * px + e1y * py + e1z * pz;
-  const double EPSILON = 0.000001;
+  const double EPSILON = rel ? 1.0e-9 * ((double)(e1x * e1x + e1y * e1y + e1z * e1z) +
+                                         (double)(e2x * e2x + e2y * e2y + e2z * e2z))
+                             : 0.000001;
   if (det > -EPSILON && det < EPSILON) return false;
   float inv_det = (float)(1.0 / det);
   float tx = ox - a.x, ty = oy - a.y, tz = oz - a.z;
@@ -302,7 +312,8 @@ struct oracle_sim {
     const V3& n = tn[tri];
     if (!(dx * n.x + dy * n.y + dz * n.z < 0.0f)) return false;
     float t, u, v;
-    if (mt_intersect(ox, oy, oz, dx, dy, dz, tv0[tri], tv1[tri], tv2[tri], t, u, v) && t >= 0.0f && t <= dist) {
+    if (mt_intersect(ox, oy, oz, dx, dy, dz, tv0[tri], tv1[tri], tv2[tri], t, u, v, true) && t >= 0.0f &&
+        t <= dist) {
       if (!found || t < best || (t == best && tri < bt)) {
         found = true;
         best = t;
@@ -973,6 +984,18 @@ int oracle_get_taus(oracle_sim* o, uint32_t* w, uint32_t n) {
     w[4 * c + 3] = o->taus[c].w;
   }
   return 0;
+}
+
+// Window (20 words, the last zero) of one MT(19,9) stream after d draws.
+void oracle_mt_window_after(uint32_t a, uint32_t b, uint32_t c, uint32_t seed, uint32_t d, uint32_t* out) {
+  MtBlock m;
+  m.a = a;
+  m.b = b;
+  m.c = c;
+  m.seed(seed);
+  for (uint32_t k = 0; k < d; ++k) m.next();
+  m.to_window(out);
+  out[19] = 0u;
 }
 
 // Draws of one Hybrid-Taus stream (the combined generator, or with

@@ -131,7 +131,8 @@ def test_gpu_stable_sort(case):
     assert np.array_equal(got_c, want_c)
     assert np.array_equal(got_p, want_p)
     valid = keys != 0xFFFFFFFF
-    assert np.array_equal(want_p, np.argsort(np.where(valid, keys, 2**32), kind="stable")[: valid.sum()])
+    assert np.array_equal(want_p, np.argsort(np.where(valid, keys.astype(np.int64), 2**32),
+                                             kind="stable")[: valid.sum()])
 
 
 # ------------------------------------------------------------ full steps
@@ -246,6 +247,8 @@ def test_gpu_overflow_tiles_bit_exact():
     s = small_reference_settings(periodic="x", dims=(16, 4, 4), ppc=40, geometry=False)
     sol = dm.Solver()
     sol.Init(s)
+    sol.set_tile_cells(256)  # one 256-cell tile of 10240 particles (the default would be 32-cell tiles)
+    assert sol.layout()["tile_cells"] == 256
     o = _mirror(sol)
     n = sol.InitSystemState((300.0, 0.0, 0.0))
     o.load((300.0, 0.0, 0.0))

@@ -145,3 +145,21 @@ def test_taus_combination_and_float():
     parts = [oracle.taus_draws(seed, 500, component=c) for c in (1, 2, 3, 4)]
     np.testing.assert_array_equal(comb, parts[0] ^ parts[1] ^ parts[2] ^ parts[3])
     assert len(set(comb.tolist())) == 500
+
+
+def test_mt_window_export_matches_the_recurrence():
+    """The oracle generates in Matsumoto-Nishimura's block form; the engine
+    keeps the last 19 words x_d .. x_{d+18} of the recurrence
+    x_{k+19} = x_{k+9} ^ twist(x_k, x_{k+1}). The exported window must be that
+    for every d, including mid-block."""
+    a, seed = 0xB40E0000 | 0x80000000, 4242
+    x = [seed]
+    for i in range(1, 19):
+        x.append((1812433253 * (x[-1] ^ (x[-1] >> 30)) + i) & 0xFFFFFFFF)
+    for k in range(80):
+        y = (x[k] & 0xFFFFFFFE) | (x[k + 1] & 1)
+        x.append(x[k + 9] ^ (y >> 1) ^ (a if y & 1 else 0))
+    for d in list(range(0, 45)) + [57, 60]:
+        w = oracle.mt_window_after(a, 0x9D2C5680, 0xEFC60000, seed, d)
+        assert w[:19].tolist() == x[d:d + 19], d
+        assert w[19] == 0
