@@ -122,7 +122,7 @@ def ensemble_bench(steps: int, warmup: int, instances: int = 64) -> dict:
     import cudadsmc_amd as dm
     from cudadsmc_amd import ensemble as ens
 
-    ns = int(ens.tuned("ensemble64", 65536))
+    ns = int(ens.tuned("ensemble64", 16384))
     cases = ens.draw_cases(instances, seed=2024, temperatures=False)
     t0 = time.perf_counter()
     sols = [ens.make_solver(c, n_streams=ns) for c in cases]
@@ -194,7 +194,7 @@ def main() -> int:
     from cudadsmc_amd import ensemble as ens
 
     s, mesh, cgrid = make_case(args.workload)
-    s.n_streams = int(ens.tuned(args.workload, 262144))
+    s.n_streams = int(ens.tuned(args.workload, 0))  # 0: the engine default
     t_setup = time.perf_counter()
     sol = dm.Solver()
     sol.Init(s)

@@ -9,7 +9,7 @@ import ctypes as C
 import os
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "lib" / "libdsmc_hip.so"
+LIB_PATH = Path(os.environ.get("DSMC_LIB") or Path(__file__).resolve().parent / "lib" / "libdsmc_hip.so")
 
 DSMC_OK = 0
 ERRORS = {-1: "INVALID", -2: "IO", -3: "DEVICE", -4: "STATE", -5: "CAPACITY", -6: "UNSUPPORTED", -7: "NUMERIC"}

@@ -137,7 +137,7 @@ def main(argv=None) -> None:
                 sol.InitSystemState(tuple(s.stream_velocity))
                 return sol
 
-            res = tune_solver(make, streams=(131072, 196608, 262144, 393216, 524288), scatter=(4096, 8192, 16384),
+            res = tune_solver(make, streams=(262144, 524288, 1048576, 2097152), scatter=(4096, 8192, 16384),
                               warmup=20, steps=40)
         else:
             raise SystemExit(f"unknown tuning target {what}")

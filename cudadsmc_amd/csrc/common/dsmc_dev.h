@@ -90,6 +90,7 @@ struct InstDesc {
   const uint32_t* cg_start;  // per cg cell: first entry (0xFFFFFFFF: none)
   const uint32_t* cg_end;
   const uint32_t* cg_tri;    // entries: triangle index
+  const float4* cg_rec;      // PHYSICS entries: {v0 + index bits in w, e1 = v1 - v0, e2 = v2 - v0, normal}
   const float4* tv0;         // per triangle vertices and unit face normal (w unused)
   const float4* tv1;
   const float4* tv2;

@@ -51,11 +51,11 @@ __device__ __forceinline__ uint32_t cg_cell(const InstDesc& I, float x, float y,
 // The reference's parallel test |det| < EPSILON is absolute (m^2), so it
 // rejects every hit on triangles of a few mm at oblique incidence; PHYSICS
 // mode (REL = true) scales the threshold with the triangle instead.
+// Edges e1 = b - a, e2 = c - a.
 template <bool REL = false>
-__device__ __forceinline__ bool intersect_tri(float ox, float oy, float oz, float dx, float dy, float dz,
-                                              float4 a, float4 b, float4 c, float& t, float& u, float& v) {
-  float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
-  float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
+__device__ __forceinline__ bool intersect_edges(float ox, float oy, float oz, float dx, float dy, float dz, float4 a,
+                                                float e1x, float e1y, float e1z, float e2x, float e2y, float e2z,
+                                                float& t, float& u, float& v) {
   float px = dy * e2z - dz * e2y;
   float py = dz * e2x - dx * e2z;
   float pz = dx * e2y - dy * e2x;
@@ -75,6 +75,13 @@ __device__ __forceinline__ bool intersect_tri(float ox, float oy, float oz, floa
   if ((double)v < 0.0 || (double)(u + v) > 1.0) return false;
   t = (e2x * qx + e2y * qy + e2z * qz) * inv_det;
   return true;
+}
+
+template <bool REL = false>
+__device__ __forceinline__ bool intersect_tri(float ox, float oy, float oz, float dx, float dy, float dz,
+                                              float4 a, float4 b, float4 c, float& t, float& u, float& v) {
+  return intersect_edges<REL>(ox, oy, oz, dx, dy, dz, a, b.x - a.x, b.y - a.y, b.z - a.z, c.x - a.x, c.y - a.y,
+                              c.z - a.z, t, u, v);
 }
 
 // normalize (vectors.cu:24-31)
@@ -273,13 +280,16 @@ __device__ __forceinline__ bool phys_nearest_hit(const InstDesc& I, float ox, fl
     uint32_t cell = (uint32_t)az.c * I.cg_dim[0] * I.cg_dim[1] + (uint32_t)ay.c * I.cg_dim[0] + (uint32_t)ax.c;
     if (I.cg_flag[cell] == kCgSurface) {
       uint32_t s = I.cg_start[cell], e = I.cg_end[cell];
-      for (uint32_t i = s; i < e; ++i) {
-        uint32_t tri = I.cg_tri[i];
+      // per-entry records {v0 + triangle id, e1, e2, n}: one contiguous
+      // 64-byte read per candidate, no dependent index loads
+      const float4* rec = I.cg_rec + 4u * s;
+      for (uint32_t i = s; i < e; ++i, rec += 4) {
+        const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], n = rec[3];
+        const uint32_t tri = __float_as_uint(r0.w);
         if (tri == skip) continue;
-        float4 n = I.tn[tri];
         if (!(dx * n.x + dy * n.y + dz * n.z < 0.0f)) continue;
         float t, u, v;
-        if (intersect_tri<true>(ox, oy, oz, dx, dy, dz, I.tv0[tri], I.tv1[tri], I.tv2[tri], t, u, v)) {
+        if (intersect_edges<true>(ox, oy, oz, dx, dy, dz, r0, r1.x, r1.y, r1.z, r2.x, r2.y, r2.z, t, u, v)) {
           if (t >= 0.0f && t <= dist) {
             if (!found || t < best || (t == best && tri < best_tri)) {
               found = true;
@@ -357,7 +367,16 @@ __device__ __forceinline__ uint32_t move_physics(const InstDesc& I, float& px, f
                                                  WallAcc& acc) {
   hits = 0;
   float tl = dt;
+  // Segments whose bounding box misses the collision grid's box cannot hit a
+  // triangle (the mesh lies at least one grid cell inside that box): skip the
+  // division-heavy clip and traversal for them.
+  bool near = false;
   if (I.has_geom) {
+    const float ex = px + vx * tl, ey = py + vy * tl, ez = pz + vz * tl;
+    near = !(fmaxf(px, ex) < I.cg_min[0] || fminf(px, ex) > I.cg_max[0] || fmaxf(py, ey) < I.cg_min[1] ||
+             fminf(py, ey) > I.cg_max[1] || fmaxf(pz, ez) < I.cg_min[2] || fminf(pz, ez) > I.cg_max[2]);
+  }
+  if (near) {
     uint32_t skip = kInvalidKey;
     for (int b = 0; b < kPhysMaxBounces; ++b) {
       float sp = sqrtf(vx * vx + vy * vy + vz * vz);

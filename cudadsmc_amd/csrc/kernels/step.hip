@@ -389,6 +389,8 @@ using dsmc_launch::kFlagSample;
 using dsmc_launch::kFlagKeepPerm;
 using dsmc_launch::kFlagHits;
 
+constexpr uint32_t kMoveBatch = 4;  // rounds of 256 positions whose loads are issued together
+
 template <uint32_t MODE, bool LDS>
 __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint32_t flags, uint32_t c0, uint32_t nc,
                                           uint32_t p0, uint32_t np, const uint32_t* s_start, uint32_t* s_raw,
@@ -435,12 +437,30 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
     srt[a + r] = v;
   }
   __syncthreads();
-  // (3) gather velocities in sorted order
-  for (uint32_t q = tid; q < np; q += kTileThreads) {
-    uint32_t src = srt[q];
-    vx[q] = ivx[src];
-    vy[q] = ivy[src];
-    vz[q] = ivz[src];
+  // (3) gather velocities in sorted order; the loads of kMoveBatch positions
+  // are issued before any store so that their latencies overlap
+  for (uint32_t q0 = tid; q0 < np; q0 += kMoveBatch * kTileThreads) {
+    float gx[kMoveBatch], gy[kMoveBatch], gz[kMoveBatch];
+#pragma unroll
+    for (uint32_t k = 0; k < kMoveBatch; ++k) {
+      const uint32_t q = q0 + k * kTileThreads;
+      gx[k] = gy[k] = gz[k] = 0.0f;
+      if (q < np) {
+        const uint32_t src = srt[q];
+        gx[k] = ivx[src];
+        gy[k] = ivy[src];
+        gz[k] = ivz[src];
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kMoveBatch; ++k) {
+      const uint32_t q = q0 + k * kTileThreads;
+      if (q < np) {
+        vx[q] = gx[k];
+        vy[q] = gy[k];
+        vz[q] = gz[k];
+      }
+    }
   }
   __syncthreads();
   // (4) NTC collisions, one lane per cell
@@ -472,12 +492,31 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
   // (5) move, walls, box faces, next key; rounds keep waves converged for
   // the aggregated counting
   const uint32_t rounds = (np + kTileThreads - 1u) / kTileThreads;
-  for (uint32_t r = 0; r < rounds; ++r) {
-    uint32_t q = r * kTileThreads + tid;
+  for (uint32_t r0 = 0; r0 < rounds; r0 += kMoveBatch) {
+    // positions of up to kMoveBatch rounds in flight before the first store
+    float bx[kMoveBatch], by[kMoveBatch], bz[kMoveBatch];
+    uint32_t bsrc[kMoveBatch];
+#pragma unroll
+    for (uint32_t k = 0; k < kMoveBatch; ++k) {
+      const uint32_t q = (r0 + k) * kTileThreads + tid;
+      bsrc[k] = 0u;
+      bx[k] = by[k] = bz[k] = 0.0f;
+      if (q < np) {
+        const uint32_t src = srt[q];
+        bsrc[k] = src;
+        bx[k] = ipx[src];
+        by[k] = ipy[src];
+        bz[k] = ipz[src];
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kMoveBatch; ++k) {
+    if (r0 + k >= rounds) break;  // uniform over the workgroup
+    uint32_t q = (r0 + k) * kTileThreads + tid;
     uint32_t key = kInvalidKey;
     if (q < np) {
-      uint32_t src = srt[q];
-      float x = ipx[src], y = ipy[src], z = ipz[src];
+      const uint32_t src = bsrc[k];
+      float x = bx[k], y = by[k], z = bz[k];
       float ux = vx[q], uy = vy[q], uz = vz[q];
       uint32_t hits;
       if (MODE == kModeReference) {
@@ -502,6 +541,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
       if (flags & kFlagKeepPerm) I.perm[p0 + q] = src;
     }
     count_keys(I.count, key);
+    }
   }
   // (6) sampling of the post-move velocities, one lane per cell
   if ((flags & kFlagSample) && I.sample_on) {
@@ -544,7 +584,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
 }
 
 template <uint32_t MODE>
-__global__ void __launch_bounds__(kTileThreads) k_tile(const InstDesc* __restrict__ Ip, uint32_t par, uint32_t flags) {
+__global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __restrict__ Ip, uint32_t par, uint32_t flags) {
   const InstDesc& I = Ip[blockIdx.y];
   if (blockIdx.x >= I.n_vwg) return;  // batched launches size the grid for the largest instance
   __shared__ uint32_t s_start[kTileCells + 1];
@@ -553,8 +593,10 @@ __global__ void __launch_bounds__(kTileThreads) k_tile(const InstDesc* __restric
   __shared__ uint8_t s_owner[kLdsCap];
   __shared__ float s_vx[kLdsCap];
   __shared__ float s_vy[kLdsCap];
-  __shared__ float s_vz[kLdsCap];
   __shared__ double s_red[kTileThreads / 64][kAccSlots];
+  // The raw permutation is dead once the rank sort has run (phase 2), so
+  // the z velocities reuse its space: 35 KB of LDS, four workgroups per CU.
+  float* const s_vz = reinterpret_cast<float*>(s_raw);
 
   const uint32_t tid = threadIdx.x;
   const uint32_t vw = blockIdx.x;

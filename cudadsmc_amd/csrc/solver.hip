@@ -278,8 +278,6 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
   } else {
     cap = s.capacity ? std::max(s.capacity, s.num_particles) : std::max<uint32_t>(s.num_particles, 1u);
   }
-  uint32_t ns = s.n_streams ? round_up(s.n_streams, kTileThreads) : 262144u;
-  if (!phys) ns = std::max(ns, round_up(std::max(C, (uint32_t)(s.grid_dim[0] * s.grid_dim[1]) * h->ppc), kTileThreads));
   // Cells per tile: keep the expected particles of a tile near half the LDS
   // capacity so that few tiles take the global-memory path.
   {
@@ -291,6 +289,19 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
     I.tile_cells = tc;
     I.ntiles = (C + tc - 1) / tc;
   }
+  // RNG streams = 256 x the tile kernel's workgroups. By default one tile
+  // per workgroup up to 1024 workgroups (small grids fill the GPU), else
+  // about eight tiles per workgroup, at most 4096 workgroups: enough
+  // workgroups to balance tiles of uneven cost (walls, dense cells) over the
+  // CUs while the per-step stream state traffic (80 B per stream, read and
+  // written) stays small next to the particle traffic.
+  uint32_t ns = s.n_streams;
+  if (!ns) {
+    uint32_t wg = std::max(std::min(I.ntiles, 1024u), (I.ntiles + 7u) / 8u);
+    ns = std::min(wg, 4096u) * kTileThreads;
+  }
+  ns = round_up(ns, kTileThreads);
+  if (!phys) ns = std::max(ns, round_up(std::max(C, (uint32_t)(s.grid_dim[0] * s.grid_dim[1]) * h->ppc), kTileThreads));
   I.cap = cap;
   I.n_streams = ns;
   I.n_vwg = ns / kTileThreads;
@@ -473,6 +484,27 @@ int dsmc_set_geometry(dsmc_solver* h, const float* verts, uint32_t nv, const uin
   HIPC(hipMemcpyAsync(t1, h1.data(), nt * sizeof(float4), hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(t2, h2.data(), nt * sizeof(float4), hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(tn, hn.data(), nt * sizeof(float4), hipMemcpyHostToDevice, st));
+  I.cg_rec = nullptr;
+  if (h->physics()) {
+    const size_t ne = std::max<size_t>(g.tri.size(), 1);
+    float4* rec;
+    CHECK_RC(h->alloc(&rec, 4 * ne, true));
+    std::vector<float4> hr(4 * ne, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (size_t i = 0; i < g.tri.size(); ++i) {
+      const uint32_t t = g.tri[i];
+      const float4 a = h0[t], b = h1[t], c = h2[t];
+      uint32_t bits = t;
+      float tf;
+      std::memcpy(&tf, &bits, 4);
+      hr[4 * i] = make_float4(a.x, a.y, a.z, tf);
+      hr[4 * i + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
+      hr[4 * i + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
+      hr[4 * i + 3] = hn[t];
+    }
+    HIPC(hipMemcpyAsync(rec, hr.data(), hr.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    HIPC(hipStreamSynchronize(st));
+    I.cg_rec = rec;
+  }
   for (int a = 0; a < 3; ++a) {
     I.cg_min[a] = g.mn[a];
     I.cg_max[a] = g.mx[a];

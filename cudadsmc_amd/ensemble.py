@@ -75,7 +75,7 @@ def draw_cases(k: int, seed: int, N_range=(1.0e4, 5.0e5), temperatures=True, see
     return out
 
 
-def settings(case: Case, dt: float = 1.0e-6, n_streams: int = 65536):
+def settings(case: Case, dt: float = 1.0e-6, n_streams: int = 0):
     s = settings_default()
     s.mode = MODE_PHYSICS
     s.bmin[:] = BOX_MIN
@@ -105,9 +105,11 @@ def collision_grid_dims(r: float) -> tuple[int, int, int]:
     return (g, g, g)
 
 
-def make_solver(case: Case, dt: float = 1.0e-6, n_streams: int = 65536, fields: dict | None = None) -> Solver:
+def make_solver(case: Case, dt: float = 1.0e-6, n_streams: int | None = None, fields: dict | None = None) -> Solver:
     """A loaded solver: cold (uniform freestream) or warm (fields n, u, v,
     w, T per cell, X fastest)."""
+    if n_streams is None:
+        n_streams = tuned("ensemble64", 16384)
     sol = Solver()
     sol.Init(settings(case, dt, n_streams))
     sol.SetModel(icosphere(CENTER, case.r, ICO_SUBDIV), grid=collision_grid_dims(case.r))
@@ -147,7 +149,7 @@ class DragWindows:
 
 
 def run_batch(cases: list[Case], transient: int, sampling: int, windows: int = 20, dt: float = 1.0e-6,
-              n_streams: int = 65536, fields: list[dict] | None = None, graphs: bool = True) -> dict:
+              n_streams: int | None = None, fields: list[dict] | None = None, graphs: bool = True) -> dict:
     """Run the cases as one batch: transient steps, then sampling steps in
     `windows` equal windows. Returns per-case fields, drag C_d (mean,
     batch-means sigma), particle counts and timings."""

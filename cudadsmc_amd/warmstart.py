@@ -220,14 +220,16 @@ def time_to_steady(cases, refs, fields, chunk: int, max_steps: int, tol: float, 
                 st["steady_step"] = step
                 st["steady_wall_s"] = wall
     b.close()
-    out = []
+    rows = []
     for st, s in zip(state, sols):
         m, e = st["drag"].cd() if st["drag"] else (float("nan"), float("nan"))
-        out.append({"steady_step": st["steady_step"], "steady_wall_s": st.get("steady_wall_s"),
-                    "sampling_from": st["sampling_from"], "final_err": st["err"], "cd": m, "cd_sigma": e,
-                    "load_s": load_s / k})
+        rows.append({"steady_step": st["steady_step"], "steady_wall_s": st.get("steady_wall_s"),
+                     "sampling_from": st["sampling_from"], "final_err": st["err"], "cd": m, "cd_sigma": e})
         s.close()
-    return out
+    done = all(r["steady_step"] is not None for r in rows)
+    # the ensemble is steady when its last member is
+    return {"rows": rows, "load_s": load_s, "steps": step, "all_steady": done,
+            "wall_to_all_steady_s": load_s + wall if done else None, "stepping_wall_s": wall}
 
 
 def evaluate(out: Path, n_cases: int, seed: int, N: int, ref_transient: int, ref_sampling: int, chunk: int,
@@ -243,22 +245,28 @@ def evaluate(out: Path, n_cases: int, seed: int, N: int, ref_transient: int, ref
     fallback = sum(1 for w in warm if w is None)
     refs = reference_runs(cases, ref_transient, ref_sampling)
     res_c = time_to_steady(cases, refs, None, chunk, max_steps, tol, block, settle, chunk)
+    print(f"[eval] cold arm: {res_c['steps']} steps, all steady: {res_c['all_steady']}", flush=True)
     res_w = time_to_steady(cases, refs, warm, chunk, max_steps, tol, block, settle, chunk)
+    print(f"[eval] warm arm: {res_w['steps']} steps, all steady: {res_w['all_steady']}", flush=True)
     rows = []
-    for c, rc, rw in zip(cases, res_c, res_w):
+    for c, rc, rw in zip(cases, res_c["rows"], res_w["rows"]):
         rows.append({"case": asdict(c), "cold": rc, "warm": rw})
     sc = [r["cold"]["steady_step"] for r in rows]
     sw = [r["warm"]["steady_step"] for r in rows]
     both = [(a, b) for a, b in zip(sc, sw) if a is not None and b is not None]
-    wc = sum(r["cold"]["steady_wall_s"] or 0 for r in rows)
-    ww = sum(r["warm"]["steady_wall_s"] or 0 for r in rows) + infer_s
+    wc = res_c["wall_to_all_steady_s"]
+    ww = None if res_w["wall_to_all_steady_s"] is None else res_w["wall_to_all_steady_s"] + infer_s
     summary = {"cases": n_cases, "N": N, "tol": tol, "block": block, "chunk": chunk, "max_steps": max_steps,
                "fallback_to_cold": fallback, "infer_s": infer_s,
                "cold_reached": sum(a is not None for a in sc), "warm_reached": sum(b is not None for b in sw),
                "median_steps_cold": float(np.median([a for a, _ in both])) if both else None,
                "median_steps_warm": float(np.median([b for _, b in both])) if both else None,
                "steps_ratio_median": float(np.median([a / b for a, b in both])) if both else None,
-               "wall_cold_s": wc, "wall_warm_s": ww}
+               "steps_all_cold": max(sc) if all(a is not None for a in sc) else None,
+               "steps_all_warm": max(sw) if all(b is not None for b in sw) else None,
+               "wall_to_all_steady_cold_s": wc, "wall_to_all_steady_warm_s": ww,
+               "wall_ratio": (wc / ww) if (wc and ww) else None,
+               "load_cold_s": res_c["load_s"], "load_warm_s": res_w["load_s"]}
     res = {"summary": summary, "rows": rows}
     (out / "eval.json").write_text(json.dumps(res, indent=1, default=float))
     print(json.dumps(summary), flush=True)
