@@ -91,6 +91,7 @@ def cpu_baseline(seconds: float = 15.0) -> dict:
     sol.SetModel(mesh, grid=cgrid)
     o = oracle.OracleSim.from_solver(sol)
     o.set_geometry(mesh, sol.collision_grid(), sol.fluid_fraction())
+    o.set_near_cells(sol.near_cells())
     o.set_rng(sol.rng_state())
     o.set_ntc(sol.ntc_state())
     sol.close()
@@ -221,10 +222,10 @@ def main() -> int:
     sol.DoBenchmark(args.steps, s.dt)
     sol.synchronize()
     elapsed = time.perf_counter() - t0
-    tm = np.zeros(6, np.float32)
+    tm = np.zeros(7, np.float32)
     import ctypes as C
 
-    dm.check(dm.lib().dsmc_get_timings(sol._h, tm.ctypes.data_as(C.POINTER(C.c_float)), 6), sol._h)
+    dm.check(dm.lib().dsmc_get_timings(sol._h, tm.ctypes.data_as(C.POINTER(C.c_float)), 7), sol._h)
     sol.set_timing(False)
     _, n_after, _ = sol.counts()
     n_avg = 0.5 * (n_before + n_after)
@@ -260,7 +261,8 @@ def main() -> int:
         "pct_hbm_peak": 100.0 * hbm_step / HBM_PEAK,
         "hbm_model": "144 B/particle-step + 32 B/cell-step (algorithmic, whole step)",
         "kernel_ms_per_step": {"scan": float(tm[0]), "scatter": float(tm[1]), "tile": float(tm[2]),
-                               "inject": float(tm[3]), "total": float(tm[4]), "timed_steps": int(tm[5])},
+                               "walls": float(tm[6]), "inject": float(tm[3]), "total": float(tm[4]),
+                               "timed_steps": int(tm[5])},
         "roofline": {"bound": "hbm", "achieved": tile_gbs / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": tile_gbs / HBM_PEAK, "traffic": None,
                      "kernel": "k_tile (56 B/particle + 20 B/cell algorithmic)"},

@@ -277,6 +277,12 @@ class Solver:
         self._c(lib().dsmc_get_ntc_state(self._h, _fp(a), self.n_cells))
         return a.reshape(-1, 2)
 
+    def near_cells(self) -> np.ndarray:
+        """PHYSICS with a body: cells whose particles go through the wall pass."""
+        a = np.zeros(self.n_cells, np.uint8)
+        self._c(lib().dsmc_get_near_cells(self._h, a.ctypes.data_as(C.POINTER(C.c_uint8)), self.n_cells))
+        return a
+
     def fluid_fraction(self) -> np.ndarray:
         a = np.zeros(self.n_cells, np.float32)
         self._c(lib().dsmc_get_fluid_fraction(self._h, _fp(a), self.n_cells))

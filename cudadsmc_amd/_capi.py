@@ -102,6 +102,7 @@ def lib() -> C.CDLL:
         "dsmc_set_tile_cells": ([P, U32], I32),
         "dsmc_get_taus_state": ([P, pU, U32], I32),
         "dsmc_set_taus_state": ([P, pU, U32], I32),
+        "dsmc_get_near_cells": ([P, C.POINTER(C.c_uint8), U32], I32),
         "dsmc_get_fluid_fraction": ([P, pF, U32], I32),
         "dsmc_get_collision_grid": ([P, pU, U32], I32),
         "dsmc_get_timings": ([P, pF, U32], I32),

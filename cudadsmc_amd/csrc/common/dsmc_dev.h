@@ -19,6 +19,7 @@
 namespace dsmc_dev {
 
 constexpr uint32_t kInvalidKey = 0xFFFFFFFFu;
+constexpr uint32_t kDeferredKey = 0xFFFFFFFEu;  // PHYSICS: moved by the wall pass
 constexpr uint32_t kTileCells = 256;    // most cells per tile (one lane per cell)
 constexpr uint32_t kTileThreads = 256;  // threads of the tile kernel
 constexpr uint32_t kLdsCap = 2048;      // particles staged in LDS per tile
@@ -115,8 +116,13 @@ struct InstDesc {
   float4* cstat;        // REFERENCE per-cell {selections, collisions, sum c_r, 0}
   double* samp;         // [5][ncells]
   float* fluid_frac;    // PHYSICS per-cell fluid volume fraction
+  const uint8_t* near_cell;  // PHYSICS with a body: cells within one cell of a collision-grid cell in use
   // ---- scalars and per-workgroup accumulators
-  uint32_t* ctr;        // [0] stored particles, [1] sorted particles, [2] injected, [3] overflow tiles
+  uint32_t* ctr;        // [0] stored, [1] sorted, [2] injected particles, [3] status bits, [4] step number,
+                        // [5] tiles queued for the wall pass
+  uint32_t* near_tiles; // PHYSICS with a body: tiles with deferred particles this step [ntiles]
+  double* tile_acc;     // PHYSICS with a body: wall {Fx, Fy, Fz, heat} sums per tile [ntiles][4]
+  uint32_t wall_key[2]; // Philox key of the wall re-emission draws
   double* wg_acc;       // [n_vwg][kAccSlots]
   uint32_t* hits;       // per sorted particle wall hits (debug, may be null)
   // ---- RNG

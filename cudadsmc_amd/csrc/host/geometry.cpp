@@ -457,6 +457,44 @@ bool point_inside(const Mesh& m, const CGrid& g, float x, float y, float z) {
   return sd < 0.0f;
 }
 
+void near_cells(const CGrid& g, const float gmin[3], const float cell[3], const uint32_t dim[3],
+                std::vector<uint8_t>& out) {
+  const uint64_t C = (uint64_t)dim[0] * dim[1] * dim[2];
+  out.assign(C, 0);
+  // collision-grid index range covered by sim cells [i - 1, i + 2) per axis
+  auto range = [&](int k, uint32_t i, int& lo, int& hi) {
+    double a = (double)gmin[k] + ((double)i - 1.0) * cell[k];
+    double b = (double)gmin[k] + ((double)i + 2.0) * cell[k];
+    lo = (int)std::floor((a - g.mn[k]) / g.cell[k]) - 1;
+    hi = (int)std::floor((b - g.mn[k]) / g.cell[k]) + 1;
+    lo = std::max(lo, 0);
+    hi = std::min(hi, (int)g.dim[k] - 1);
+  };
+  std::vector<uint8_t> occ(g.ncells());
+  for (uint32_t c = 0; c < g.ncells(); ++c) occ[c] = g.flag[c] != 0u;
+  for (uint32_t z = 0; z < dim[2]; ++z) {
+    int z0, z1;
+    range(2, z, z0, z1);
+    if (z0 > z1) continue;
+    for (uint32_t y = 0; y < dim[1]; ++y) {
+      int y0, y1;
+      range(1, y, y0, y1);
+      if (y0 > y1) continue;
+      for (uint32_t x = 0; x < dim[0]; ++x) {
+        int x0, x1;
+        range(0, x, x0, x1);
+        if (x0 > x1) continue;
+        bool hit = false;
+        for (int cz = z0; cz <= z1 && !hit; ++cz)
+          for (int cy = y0; cy <= y1 && !hit; ++cy)
+            for (int cx = x0; cx <= x1 && !hit; ++cx)
+              hit = occ[(uint64_t)cz * g.dim[0] * g.dim[1] + (uint64_t)cy * g.dim[0] + cx] != 0;
+        out[(uint64_t)z * dim[0] * dim[1] + (uint64_t)y * dim[0] + x] = hit ? 1 : 0;
+      }
+    }
+  }
+}
+
 void fluid_fractions(const Mesh& m, const CGrid& g, const float gmin[3], const float cell[3], const uint32_t dim[3],
                      std::vector<float>& frac) {
   const uint64_t C = (uint64_t)dim[0] * dim[1] * dim[2];

@@ -57,6 +57,12 @@ bool tri_box_overlap(const float center[3], const float half[3], const float a[3
 bool point_inside(const Mesh& m, const CGrid& g, float x, float y, float z);
 
 // Fluid volume fraction of each simulation cell (4x4x4 stratified points).
+// PHYSICS: 1 for every simulation cell whose box, widened by one cell on
+// each side, meets a collision-grid cell holding triangles or interior. A
+// particle in an unflagged cell that moves less than one cell per axis in a
+// step cannot reach a triangle.
+void near_cells(const CGrid& g, const float gmin[3], const float cell[3], const uint32_t dim[3],
+                std::vector<uint8_t>& out);
 void fluid_fractions(const Mesh& m, const CGrid& g, const float gmin[3], const float cell[3], const uint32_t dim[3],
                      std::vector<float>& frac);
 

@@ -32,6 +32,9 @@ void scatter_keys(const uint32_t* keys, uint32_t n, uint32_t* cursor, uint32_t* 
 void fix_segments(const uint32_t* start, uint32_t n_cells, uint32_t* perm, hipStream_t st);
 void tile(uint32_t mode, const InstDesc* d, uint32_t par, uint32_t flags, uint32_t n_vwg, hipStream_t st,
           uint32_t n_inst = 1);
+// PHYSICS wall pass over the tiles the tile kernel queued (opar = the
+// buffer the tile kernel wrote); `grid` workgroups per instance.
+void walls(const InstDesc* d, uint32_t opar, uint32_t flags, uint32_t grid, hipStream_t st, uint32_t n_inst = 1);
 void load_reference(const InstDesc* d, uint32_t par, uint32_t ppc, uint32_t n_threads, float sux, float suy,
                     float suz, float temp, hipStream_t st);
 void move_zero(const InstDesc* d, uint32_t par, uint32_t n, hipStream_t st);

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../common/dsmc_dev.h"
+#include "../common/philox.h"
 
 namespace dsmc_dev {
 
@@ -261,6 +262,8 @@ __device__ __forceinline__ DdaAxis dda_axis(float o, float d, float t0, float lo
   return r;
 }
 
+constexpr uint32_t kRecBatch = 4;  // triangle records loaded together by the wall pass
+
 __device__ __forceinline__ bool phys_nearest_hit(const InstDesc& I, float ox, float oy, float oz, float dx,
                                                  float dy, float dz, float dist, uint32_t skip,
                                                  float& s_hit, uint32_t& tri_hit) {
@@ -283,18 +286,31 @@ __device__ __forceinline__ bool phys_nearest_hit(const InstDesc& I, float ox, fl
       // per-entry records {v0 + triangle id, e1, e2, n}: one contiguous
       // 64-byte read per candidate, no dependent index loads
       const float4* rec = I.cg_rec + 4u * s;
-      for (uint32_t i = s; i < e; ++i, rec += 4) {
-        const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], n = rec[3];
-        const uint32_t tri = __float_as_uint(r0.w);
-        if (tri == skip) continue;
-        if (!(dx * n.x + dy * n.y + dz * n.z < 0.0f)) continue;
-        float t, u, v;
-        if (intersect_edges<true>(ox, oy, oz, dx, dy, dz, r0, r1.x, r1.y, r1.z, r2.x, r2.y, r2.z, t, u, v)) {
-          if (t >= 0.0f && t <= dist) {
-            if (!found || t < best || (t == best && tri < best_tri)) {
-              found = true;
-              best = t;
-              best_tri = tri;
+      // kRecBatch records in flight before any is tested
+      for (uint32_t i = s; i < e; i += kRecBatch, rec += 4 * kRecBatch) {
+        float4 ra[kRecBatch][4];
+#pragma unroll
+        for (uint32_t k = 0; k < kRecBatch; ++k) {
+          if (i + k < e) {
+#pragma unroll
+            for (int w = 0; w < 4; ++w) ra[k][w] = rec[4 * k + w];
+          }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kRecBatch; ++k) {
+          if (i + k >= e) break;
+          const float4 r0 = ra[k][0], r1 = ra[k][1], r2 = ra[k][2], n = ra[k][3];
+          const uint32_t tri = __float_as_uint(r0.w);
+          if (tri == skip) continue;
+          if (!(dx * n.x + dy * n.y + dz * n.z < 0.0f)) continue;
+          float t, u, v;
+          if (intersect_edges<true>(ox, oy, oz, dx, dy, dz, r0, r1.x, r1.y, r1.z, r2.x, r2.y, r2.z, t, u, v)) {
+            if (t >= 0.0f && t <= dist) {
+              if (!found || t < best || (t == best && tri < best_tri)) {
+                found = true;
+                best = t;
+                best_tri = tri;
+              }
             }
           }
         }
@@ -359,81 +375,103 @@ struct WallAcc {
   double fx, fy, fz, heat;
 };
 
-// PHYSICS move: free flight with nearest-hit walls (diffuse at wall_temp or
-// specular), then the box faces. Returns the new cell key or kInvalidKey
-// when the particle left through a freestream face.
-__device__ __forceinline__ uint32_t move_physics(const InstDesc& I, float& px, float& py, float& pz, float& vx,
-                                                 float& vy, float& vz, float dt, Mt& rng, uint32_t& hits,
-                                                 WallAcc& acc) {
+// PHYSICS move. A particle is "near" when its cell is flagged as within one
+// cell of the body's collision-grid cells, or when it moves a cell or more
+// along some axis in one step: its wall interaction is deferred to the wall
+// pass (k_walls), which runs after the tile kernel with the particle's state
+// as left there. Every other particle can reach no triangle and only flies;
+// for a near particle that hits nothing the wall pass computes exactly the
+// same flight, so the split changes no result.
+__device__ __forceinline__ bool phys_near(const InstDesc& I, bool near_cell, float vx, float vy, float vz,
+                                          float dt) {
+  if (!I.has_geom) return false;
+  return near_cell || fabsf(vx) * dt >= I.cell[0] || fabsf(vy) * dt >= I.cell[1] || fabsf(vz) * dt >= I.cell[2];
+}
+
+// Box faces after the flight: the new cell key, or kInvalidKey when the
+// particle left through a freestream face.
+__device__ __forceinline__ uint32_t phys_faces_key(const InstDesc& I, float& px, float& py, float& pz, float& vx,
+                                                   float& vy, float& vz) {
+  if (!phys_face(px, vx, I.gmin[0], I.gmax[0], I.face_bc[0], I.face_bc[1])) return kInvalidKey;
+  if (!phys_face(py, vy, I.gmin[1], I.gmax[1], I.face_bc[2], I.face_bc[3])) return kInvalidKey;
+  if (!phys_face(pz, vz, I.gmin[2], I.gmax[2], I.face_bc[4], I.face_bc[5])) return kInvalidKey;
+  return cell_key(I, px, py, pz);
+}
+
+// Free flight of a particle that is not near the body.
+__device__ __forceinline__ uint32_t move_free(const InstDesc& I, float& px, float& py, float& pz, float& vx,
+                                              float& vy, float& vz, float dt) {
+  px = px + vx * dt;
+  py = py + vy * dt;
+  pz = pz + vz * dt;
+  return phys_faces_key(I, px, py, pz, vx, vy, vz);
+}
+
+// Flight of a near particle: nearest-hit walls (diffuse at wall_temp, else
+// specular), up to kPhysMaxBounces, then the box faces. The diffuse
+// re-emission of bounce b draws Philox4x32-10 at counter (slot, step, 0, b)
+// under key (seed, instance salt): the draws depend on which particle
+// bounces when, not on which thread handles it.
+__device__ __forceinline__ uint32_t move_walls(const InstDesc& I, float& px, float& py, float& pz, float& vx,
+                                               float& vy, float& vz, float dt, uint32_t slot, uint32_t stepno,
+                                               uint32_t& hits, WallAcc& acc) {
   hits = 0;
   float tl = dt;
-  // Segments whose bounding box misses the collision grid's box cannot hit a
-  // triangle (the mesh lies at least one grid cell inside that box): skip the
-  // division-heavy clip and traversal for them.
-  bool near = false;
-  if (I.has_geom) {
-    const float ex = px + vx * tl, ey = py + vy * tl, ez = pz + vz * tl;
-    near = !(fmaxf(px, ex) < I.cg_min[0] || fminf(px, ex) > I.cg_max[0] || fmaxf(py, ey) < I.cg_min[1] ||
-             fminf(py, ey) > I.cg_max[1] || fmaxf(pz, ez) < I.cg_min[2] || fminf(pz, ez) > I.cg_max[2]);
-  }
-  if (near) {
-    uint32_t skip = kInvalidKey;
-    for (int b = 0; b < kPhysMaxBounces; ++b) {
-      float sp = sqrtf(vx * vx + vy * vy + vz * vz);
-      float dist = sp * tl;
-      if (!(dist > 0.0f)) break;
-      float dx = vx / sp, dy = vy / sp, dz = vz / sp;
-      float s;
-      uint32_t tri;
-      if (!phys_nearest_hit(I, px, py, pz, dx, dy, dz, dist, skip, s, tri)) break;
-      float4 n = I.tn[tri];
-      float hx = px + dx * s, hy = py + dy * s, hz = pz + dz * s;
-      float ux, uy, uz;
-      if (I.vm_wall > 0.0f) {
-        float r1 = mt_float(rng), r2 = mt_float(rng), r3 = mt_float(rng);
-        float vn = I.vm_wall * sqrtf(-dm_logf(r1));
-        float rt = I.vm_wall * sqrtf(-dm_logf(r2));
-        float sn, cs;
-        dm_sincosf(6.28318530717958647692f * r3, &sn, &cs);
-        float a1 = rt * cs, a2 = rt * sn;
-        float t1[3], t2[3];
-        tangents(n.x, n.y, n.z, t1, t2);
-        ux = n.x * vn + t1[0] * a1 + t2[0] * a2;
-        uy = n.y * vn + t1[1] * a1 + t2[1] * a2;
-        uz = n.z * vn + t1[2] * a1 + t2[2] * a2;
-      } else {
-        float dd = vx * n.x + vy * n.y + vz * n.z;
-        ux = vx - 2.0f * dd * n.x;
-        uy = vy - 2.0f * dd * n.y;
-        uz = vz - 2.0f * dd * n.z;
-      }
-      double m = (double)I.mass;
-      acc.fx += m * ((double)vx - (double)ux);
-      acc.fy += m * ((double)vy - (double)uy);
-      acc.fz += m * ((double)vz - (double)uz);
-      acc.heat += 0.5 * m * (((double)vx * vx + (double)vy * vy + (double)vz * vz) -
-                             ((double)ux * ux + (double)uy * uy + (double)uz * uz));
-      ++hits;
-      tl = tl - s / sp;
-      const float off = 1.0e-6f;
-      px = hx + n.x * off;
-      py = hy + n.y * off;
-      pz = hz + n.z * off;
-      vx = ux;
-      vy = uy;
-      vz = uz;
-      skip = tri;
+  uint32_t skip = kInvalidKey;
+  for (int b = 0; b < kPhysMaxBounces; ++b) {
+    float sp = sqrtf(vx * vx + vy * vy + vz * vz);
+    float dist = sp * tl;
+    if (!(dist > 0.0f)) break;
+    float dx = vx / sp, dy = vy / sp, dz = vz / sp;
+    float s;
+    uint32_t tri;
+    if (!phys_nearest_hit(I, px, py, pz, dx, dy, dz, dist, skip, s, tri)) break;
+    float4 n = I.tn[tri];
+    float hx = px + dx * s, hy = py + dy * s, hz = pz + dz * s;
+    float ux, uy, uz;
+    if (I.vm_wall > 0.0f) {
+      dsmc_rng::U32x4 c = {{slot, stepno, 0u, (uint32_t)b}};
+      dsmc_rng::U32x4 r = dsmc_rng::philox4x32_10(c, I.wall_key[0], I.wall_key[1]);
+      float r1 = dsmc_rng::px_unit(r.v[0]), r2 = dsmc_rng::px_unit(r.v[1]), r3 = dsmc_rng::px_unit(r.v[2]);
+      float vn = I.vm_wall * sqrtf(-dm_logf(r1));
+      float rt = I.vm_wall * sqrtf(-dm_logf(r2));
+      float sn, cs;
+      dm_sincosf(6.28318530717958647692f * r3, &sn, &cs);
+      float a1 = rt * cs, a2 = rt * sn;
+      float t1[3], t2[3];
+      tangents(n.x, n.y, n.z, t1, t2);
+      ux = n.x * vn + t1[0] * a1 + t2[0] * a2;
+      uy = n.y * vn + t1[1] * a1 + t2[1] * a2;
+      uz = n.z * vn + t1[2] * a1 + t2[2] * a2;
+    } else {
+      float dd = vx * n.x + vy * n.y + vz * n.z;
+      ux = vx - 2.0f * dd * n.x;
+      uy = vy - 2.0f * dd * n.y;
+      uz = vz - 2.0f * dd * n.z;
     }
+    double m = (double)I.mass;
+    acc.fx += m * ((double)vx - (double)ux);
+    acc.fy += m * ((double)vy - (double)uy);
+    acc.fz += m * ((double)vz - (double)uz);
+    acc.heat += 0.5 * m * (((double)vx * vx + (double)vy * vy + (double)vz * vz) -
+                           ((double)ux * ux + (double)uy * uy + (double)uz * uz));
+    ++hits;
+    tl = tl - s / sp;
+    const float off = 1.0e-6f;
+    px = hx + n.x * off;
+    py = hy + n.y * off;
+    pz = hz + n.z * off;
+    vx = ux;
+    vy = uy;
+    vz = uz;
+    skip = tri;
   }
   if (tl > 0.0f) {
     px = px + vx * tl;
     py = py + vy * tl;
     pz = pz + vz * tl;
   }
-  if (!phys_face(px, vx, I.gmin[0], I.gmax[0], I.face_bc[0], I.face_bc[1])) return kInvalidKey;
-  if (!phys_face(py, vy, I.gmin[1], I.gmax[1], I.face_bc[2], I.face_bc[3])) return kInvalidKey;
-  if (!phys_face(pz, vz, I.gmin[2], I.gmax[2], I.face_bc[4], I.face_bc[5])) return kInvalidKey;
-  return cell_key(I, px, py, pz);
+  return phys_faces_key(I, px, py, pz, vx, vy, vz);
 }
 
 // ------------------------------------------------------------------- NTC

@@ -280,7 +280,11 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_tiles(const InstDesc* __
   }
   if (threadIdx.x == 0) {
     ts[dsmc_launch::kScanMaxTiles] = total;
-    if (which == 0) I.ctr[1] = total;
+    if (which == 0) {
+      I.ctr[1] = total;
+      I.ctr[4] += 1u;  // step number (counter of the wall draws)
+      I.ctr[5] = 0u;   // tiles queued for the wall pass
+    }
   }
 }
 
@@ -392,10 +396,11 @@ using dsmc_launch::kFlagHits;
 constexpr uint32_t kMoveBatch = 4;  // rounds of 256 positions whose loads are issued together
 
 template <uint32_t MODE, bool LDS>
-__device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint32_t flags, uint32_t c0, uint32_t nc,
-                                          uint32_t p0, uint32_t np, const uint32_t* s_start, uint32_t* s_raw,
+__device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint32_t flags, uint32_t tile,
+                                          uint32_t c0, uint32_t nc, uint32_t p0, uint32_t np,
+                                          const uint32_t* s_start, const uint8_t* s_nearc, uint32_t* s_raw,
                                           uint32_t* s_perm, uint8_t* s_owner, float* s_vx, float* s_vy,
-                                          float* s_vz, Mt& rng, CollAcc& cacc, WallAcc& wacc) {
+                                          float* s_vz, Mt& rng, CollAcc& cacc) {
   const uint32_t tid = threadIdx.x;
   const uint32_t opar = par ^ 1u;
   const float* __restrict__ ipx = I.px[par];
@@ -492,6 +497,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
   // (5) move, walls, box faces, next key; rounds keep waves converged for
   // the aggregated counting
   const uint32_t rounds = (np + kTileThreads - 1u) / kTileThreads;
+  int any_near = 0;
   for (uint32_t r0 = 0; r0 < rounds; r0 += kMoveBatch) {
     // positions of up to kMoveBatch rounds in flight before the first store
     float bx[kMoveBatch], by[kMoveBatch], bz[kMoveBatch];
@@ -518,12 +524,15 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
       const uint32_t src = bsrc[k];
       float x = bx[k], y = by[k], z = bz[k];
       float ux = vx[q], uy = vy[q], uz = vz[q];
-      uint32_t hits;
+      uint32_t hits = 0;
       if (MODE == kModeReference) {
         hits = move_reference(I, x, y, z, ux, uy, uz, I.dt);
         key = cell_key(I, x, y, z);
+      } else if (phys_near(I, s_nearc[own[q]] != 0, ux, uy, uz, I.dt)) {
+        key = kDeferredKey;  // stored as is; k_walls moves it and counts its key
+        any_near = 1;
       } else {
-        key = move_physics(I, x, y, z, ux, uy, uz, I.dt, rng, hits, wacc);
+        key = move_free(I, x, y, z, ux, uy, uz, I.dt);
       }
       opx[q] = x;
       opy[q] = y;
@@ -540,8 +549,12 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
       if (flags & kFlagHits) I.hits[p0 + q] = hits;
       if (flags & kFlagKeepPerm) I.perm[p0 + q] = src;
     }
-    count_keys(I.count, key);
+    count_keys(I.count, key == kDeferredKey ? kInvalidKey : key);
     }
+  }
+  if (MODE == kModePhysics) {
+    // queue the tile for the wall pass when any of its particles is near
+    if (__syncthreads_or(any_near) && tid == 0) I.near_tiles[atomicAdd(&I.ctr[5], 1u)] = tile;
   }
   // (6) sampling of the post-move velocities, one lane per cell
   if ((flags & kFlagSample) && I.sample_on) {
@@ -588,6 +601,7 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
   const InstDesc& I = Ip[blockIdx.y];
   if (blockIdx.x >= I.n_vwg) return;  // batched launches size the grid for the largest instance
   __shared__ uint32_t s_start[kTileCells + 1];
+  __shared__ uint8_t s_nearc[kTileCells];
   __shared__ uint32_t s_raw[kLdsCap];
   __shared__ uint32_t s_perm[kLdsCap];
   __shared__ uint8_t s_owner[kLdsCap];
@@ -604,9 +618,6 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
   Mt rng;
   mt_load(I, stream, rng);
   CollAcc cacc = {0.0, 0.0, 0.0};
-  WallAcc wacc = {0.0, 0.0, 0.0, 0.0};
-  double hits_acc = 0.0;
-  (void)hits_acc;
   if (vw == 0 && tid == 0) I.ctr[0] = I.ctr[1];
 
   for (uint32_t tile = vw; tile < I.ntiles; tile += I.n_vwg) {
@@ -614,21 +625,22 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
     const uint32_t nc = min(I.tile_cells, I.ncells - c0);
     s_start[tid] = I.cell_start[c0 + min(tid, nc)];
     if (tid == 0) s_start[nc] = I.cell_start[c0 + nc];
+    if (MODE == kModePhysics) s_nearc[tid] = (I.near_cell && tid < nc) ? I.near_cell[c0 + tid] : (uint8_t)0;
     __syncthreads();
     const uint32_t p0 = s_start[0];
     const uint32_t np = s_start[nc] - p0;
     if (np <= kLdsCap)
-      tile_body<MODE, true>(I, par, flags, c0, nc, p0, np, s_start, s_raw, s_perm, s_owner, s_vx, s_vy, s_vz, rng,
-                            cacc, wacc);
+      tile_body<MODE, true>(I, par, flags, tile, c0, nc, p0, np, s_start, s_nearc, s_raw, s_perm, s_owner, s_vx,
+                            s_vy, s_vz, rng, cacc);
     else
-      tile_body<MODE, false>(I, par, flags, c0, nc, p0, np, s_start, s_raw, s_perm, s_owner, s_vx, s_vy, s_vz,
-                             rng, cacc, wacc);
+      tile_body<MODE, false>(I, par, flags, tile, c0, nc, p0, np, s_start, s_nearc, s_raw, s_perm, s_owner, s_vx,
+                             s_vy, s_vz, rng, cacc);
     __syncthreads();
   }
   mt_store(I, stream, rng);
 
   // per-workgroup accumulators (deterministic: fixed reduction order)
-  double v[kAccSlots] = {cacc.sel, cacc.coll, cacc.cr, wacc.fx, wacc.fy, wacc.fz, wacc.heat, 0.0};
+  double v[kAccSlots] = {cacc.sel, cacc.coll, cacc.cr, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int k = 0; k < kAccSlots; ++k) {
     double x = v[k];
@@ -645,6 +657,94 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
 
 template __global__ void k_tile<kModeReference>(const InstDesc*, uint32_t, uint32_t);
 template __global__ void k_tile<kModePhysics>(const InstDesc*, uint32_t, uint32_t);
+
+// =============================================================== wall pass
+// PHYSICS: the particles the tile kernel deferred (key kDeferredKey, state
+// after collisions, before the move) fly with walls and faces here. One
+// workgroup per queued tile at a time: the deferred positions of the tile
+// are compacted in increasing order in LDS and shared out to all 256 lanes,
+// so the divergent traversal runs on full waves; the kernel holds no RNG
+// state (Philox draws) and little else, and runs at high occupancy. Wall
+// forces are summed per thread in a fixed order, reduced in a fixed order
+// and added to the tile's own accumulator (deterministic for any schedule).
+constexpr uint32_t kWallScan = 1024;  // positions compacted per pass
+
+__global__ void __launch_bounds__(kTileThreads) k_walls(const InstDesc* __restrict__ Ip, uint32_t opar,
+                                                       uint32_t flags) {
+  const InstDesc& I = Ip[blockIdx.y];
+  if (!I.near_tiles) return;
+  __shared__ uint32_t s_q[kWallScan];
+  __shared__ uint32_t s_cnt[kTileThreads / 64];
+  __shared__ double s_red[kTileThreads / 64][4];
+  const uint32_t ntl = I.ctr[5];
+  const uint32_t stepno = I.ctr[4];
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / 64;
+  float* __restrict__ px = I.px[opar];
+  float* __restrict__ py = I.py[opar];
+  float* __restrict__ pz = I.pz[opar];
+  float* __restrict__ vx = I.vx[opar];
+  float* __restrict__ vy = I.vy[opar];
+  float* __restrict__ vz = I.vz[opar];
+  uint32_t* __restrict__ key = I.key[opar];
+  for (uint32_t j = blockIdx.x; j < ntl; j += gridDim.x) {
+    const uint32_t tile = I.near_tiles[j];
+    const uint32_t c0 = tile * I.tile_cells;
+    const uint32_t nc = min(I.tile_cells, I.ncells - c0);
+    const uint32_t p0 = I.cell_start[c0];
+    const uint32_t np = I.cell_start[c0 + nc] - p0;
+    WallAcc acc = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t b0 = 0; b0 < np; b0 += kWallScan) {
+      const uint32_t bn = min(kWallScan, np - b0);
+      uint32_t cnt = 0;
+      for (uint32_t r = 0; r < bn; r += kTileThreads) {
+        const uint32_t q = b0 + r + tid;
+        const bool d = (r + tid < bn) && key[p0 + q] == kDeferredKey;
+        const uint64_t m = __ballot(d);
+        if (lane == 0) s_cnt[wid] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = cnt, total = 0;
+        for (uint32_t w = 0; w < kTileThreads / 64; ++w) {
+          if (w < wid) before += s_cnt[w];
+          total += s_cnt[w];
+        }
+        if (d) s_q[before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = q;
+        cnt += total;
+        __syncthreads();
+      }
+      for (uint32_t i = tid; i < cnt; i += kTileThreads) {
+        const uint32_t slot = p0 + s_q[i];
+        float x = px[slot], y = py[slot], z = pz[slot];
+        float ux = vx[slot], uy = vy[slot], uz = vz[slot];
+        uint32_t hits;
+        const uint32_t k = move_walls(I, x, y, z, ux, uy, uz, I.dt, slot, stepno, hits, acc);
+        px[slot] = x;
+        py[slot] = y;
+        pz[slot] = z;
+        vx[slot] = ux;
+        vy[slot] = uy;
+        vz[slot] = uz;
+        key[slot] = k;
+        if (k != kInvalidKey) atomicAdd(&I.count[k], 1u);
+        if (flags & kFlagHits) I.hits[slot] = hits;
+      }
+      __syncthreads();
+    }
+    double v[4] = {acc.fx, acc.fy, acc.fz, acc.heat};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double x = v[k];
+      for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+      if (lane == 0) s_red[wid][k] = x;
+    }
+    __syncthreads();
+    if (tid < 4) {
+      double t = 0.0;
+      for (uint32_t w = 0; w < kTileThreads / 64; ++w) t += s_red[w][tid];
+      I.tile_acc[(size_t)tile * 4 + tid] += t;
+    }
+    __syncthreads();
+  }
+}
 
 // =================================================================== loading
 // GenerateInitialVelocitiesDevBlocks / GenerateUniformVelocitiesDevBlocks
@@ -1021,6 +1121,10 @@ void scatter_keys(const uint32_t* keys, uint32_t n, uint32_t* cursor, uint32_t* 
 void fix_segments(const uint32_t* start, uint32_t n_cells, uint32_t* perm, hipStream_t st) {
   if (!n_cells) return;
   hipLaunchKernelGGL(k_fix_segments, dim3(blocks_for(n_cells, 256)), dim3(256), 0, st, start, n_cells, perm);
+}
+
+void walls(const InstDesc* d, uint32_t opar, uint32_t flags, uint32_t grid, hipStream_t st, uint32_t n_inst) {
+  hipLaunchKernelGGL(k_walls, dim3(grid, n_inst), dim3(kTileThreads), 0, st, d, opar, flags);
 }
 
 void tile(uint32_t mode, const InstDesc* d, uint32_t par, uint32_t flags, uint32_t n_vwg, hipStream_t st,

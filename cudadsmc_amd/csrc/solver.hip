@@ -50,6 +50,7 @@ struct dsmc_solver {
   CGrid cg;
   bool have_geom = false;
   std::vector<float> frac;
+  std::vector<uint8_t> nearc;  // PHYSICS near-body cell flags
   uint32_t par = 0;
   uint64_t steps = 0;
   uint32_t samples = 0;
@@ -61,7 +62,7 @@ struct dsmc_solver {
   uint32_t ppc = 0;
   float ntc_init = 0.0f;
   bool timing = false;
-  double t_ms[5] = {0, 0, 0, 0, 0};  // summed milliseconds per kernel class
+  double t_ms[6] = {0, 0, 0, 0, 0, 0};  // summed ms: scan, scatter, tile, inject, whole step, walls
   uint64_t timed_steps = 0;
   std::vector<hipEvent_t> ev_pool;   // 5 events per timed step, resolved lazily
   size_t ev_used = 0;
@@ -199,6 +200,7 @@ const char* dsmc_last_error(const dsmc_solver* h) { return h ? h->err.c_str() : 
 
 static int physics_fnum(dsmc_solver* h);
 static int setup_injection(dsmc_solver* h);
+static int sum_acc(dsmc_solver* h, double out[kAccSlots]);
 
 int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
   if (!h || !sp) return set_global_error(DSMC_ERR_INVALID, "null argument");
@@ -233,6 +235,8 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
   I.periodic = (uint32_t)(unsigned char)s.periodic_condition;
   for (int f = 0; f < 6; ++f) I.face_bc[f] = (uint32_t)s.face_bc[f];
   I.wall_temp = s.wall_temp;
+  I.wall_key[0] = s.seed;
+  I.wall_key[1] = 0x6A09E667u;
   I.vm_wall = s.wall_temp > 0.0f ? (float)std::sqrt(2.0 * kBoltz * s.wall_temp / s.molmass) : 0.0f;
   // InitSimulationProperties (dsmc_kernel.cu:309-348), FP32 as written there
   I.sigma_hs = kPiRef * s.diameter * s.diameter;
@@ -528,6 +532,11 @@ int dsmc_set_geometry(dsmc_solver* h, const float* verts, uint32_t nv, const uin
     CHECK_RC(h->alloc(&df, I.ncells, true));
     HIPC(hipMemcpyAsync(df, h->frac.data(), (size_t)I.ncells * 4, hipMemcpyHostToDevice, st));
     I.fluid_frac = df;
+    near_cells(g, I.gmin, I.cell, I.dim, h->nearc);
+    uint8_t* dn;
+    CHECK_RC(h->alloc(&dn, I.ncells, true));
+    HIPC(hipMemcpyAsync(dn, h->nearc.data(), (size_t)I.ncells, hipMemcpyHostToDevice, st));
+    I.near_cell = dn;
     // F_n and the inflow per face depend on the fluid volume
     CHECK_RC(physics_fnum(h));
     CHECK_RC(setup_injection(h));
@@ -540,6 +549,13 @@ int dsmc_set_geometry(dsmc_solver* h, const float* verts, uint32_t nv, const uin
 static int finish_loading(dsmc_solver* h, uint32_t n, uint32_t* n_out) {
   InstDesc& I = h->h;
   hipStream_t st = h->stream;
+  if (h->physics() && I.has_geom && !I.near_tiles) {
+    // wall pass queue and per-tile wall force sums (tile count is fixed now)
+    CHECK_RC(h->alloc(&I.near_tiles, I.ntiles));
+    CHECK_RC(h->alloc(&I.tile_acc, (size_t)I.ntiles * 4));
+    HIPC(hipMemsetAsync(I.tile_acc, 0, (size_t)I.ntiles * 4 * sizeof(double), st));
+    CHECK_RC(h->upload_desc());
+  }
   uint32_t ctr[2] = {n, n};
   HIPC(hipMemcpyAsync(I.ctr, ctr, 8, hipMemcpyHostToDevice, st));
   HIPC(hipMemsetAsync(I.count, 0, (size_t)I.ncells * 4, st));
@@ -699,14 +715,7 @@ int dsmc_set_sampling(dsmc_solver* h, int enabled, int reset) {
     HIPC(hipMemsetAsync(I.samp, 0, (size_t)5 * I.ncells * 8, h->stream));
     h->samples = 0;
     h->sample_steps0 = h->steps;
-    std::vector<double> acc((size_t)I.n_vwg * kAccSlots);
-    HIPC(hipMemcpyAsync(acc.data(), I.wg_acc, acc.size() * 8, hipMemcpyDeviceToHost, h->stream));
-    HIPC(hipStreamSynchronize(h->stream));
-    for (int k = 0; k < kAccSlots; ++k) {
-      double t = 0.0;
-      for (uint32_t w = 0; w < I.n_vwg; ++w) t += acc[(size_t)w * kAccSlots + k];
-      h->acc_snap[k] = t;
-    }
+    CHECK_RC(sum_acc(h, h->acc_snap));
   }
   return DSMC_OK;
 }
@@ -781,11 +790,19 @@ static int sum_acc(dsmc_solver* h, double out[kAccSlots]) {
   InstDesc& I = h->h;
   std::vector<double> acc((size_t)I.n_vwg * kAccSlots);
   HIPC(hipMemcpyAsync(acc.data(), I.wg_acc, acc.size() * 8, hipMemcpyDeviceToHost, h->stream));
+  std::vector<double> tacc(I.tile_acc ? (size_t)I.ntiles * 4 : 0);
+  if (I.tile_acc) HIPC(hipMemcpyAsync(tacc.data(), I.tile_acc, tacc.size() * 8, hipMemcpyDeviceToHost, h->stream));
   HIPC(hipStreamSynchronize(h->stream));
   for (int k = 0; k < kAccSlots; ++k) {
     double t = 0.0;
     for (uint32_t w = 0; w < I.n_vwg; ++w) t += acc[(size_t)w * kAccSlots + k];
     out[k] = t;
+  }
+  // wall forces and heat of the wall pass, summed over tiles in order
+  for (int k = 0; k < 4 && I.tile_acc; ++k) {
+    double t = 0.0;
+    for (uint32_t tl = 0; tl < I.ntiles; ++tl) t += tacc[(size_t)tl * 4 + k];
+    out[kAccFx + k] += t;
   }
   return DSMC_OK;
 }
@@ -1070,6 +1087,14 @@ int dsmc_set_taus_state(dsmc_solver* h, const uint32_t* in, uint32_t n) {
   return DSMC_OK;
 }
 
+int dsmc_get_near_cells(dsmc_solver* h, uint8_t* out, uint32_t n) {
+  if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  InstDesc& I = h->h;
+  if (n < I.ncells) return h->fail(DSMC_ERR_INVALID, "need %u cells", I.ncells);
+  for (uint32_t c = 0; c < I.ncells; ++c) out[c] = h->nearc.empty() ? 0 : h->nearc[c];
+  return DSMC_OK;
+}
+
 int dsmc_get_fluid_fraction(dsmc_solver* h, float* out, uint32_t n) {
   if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
   InstDesc& I = h->h;
@@ -1090,13 +1115,16 @@ int dsmc_get_collision_grid(dsmc_solver* h, uint32_t* flags, uint32_t n) {
 static int fold_timings(dsmc_solver* h) {
   if (h->ev_used == 0) return DSMC_OK;
   HIPC(hipEventSynchronize(h->ev_pool[h->ev_used - 1]));
-  for (size_t g = 0; g + 5 <= h->ev_used; g += 5) {
+  // events per step: start, after scan, after scatter, after tile, after
+  // walls, after injection
+  static const int slot_of[5] = {0, 1, 2, 5, 3};
+  for (size_t g = 0; g + 6 <= h->ev_used; g += 6) {
     float t;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 5; ++k) {
       HIPC(hipEventElapsedTime(&t, h->ev_pool[g + k], h->ev_pool[g + k + 1]));
-      h->t_ms[k] += t;
+      h->t_ms[slot_of[k]] += t;
     }
-    HIPC(hipEventElapsedTime(&t, h->ev_pool[g], h->ev_pool[g + 4]));
+    HIPC(hipEventElapsedTime(&t, h->ev_pool[g], h->ev_pool[g + 5]));
     h->t_ms[4] += t;
     h->timed_steps += 1;
   }
@@ -1110,6 +1138,7 @@ int dsmc_get_timings(dsmc_solver* h, float* out_ms, uint32_t n) {
   double d = h->timed_steps ? (double)h->timed_steps : 1.0;
   for (uint32_t k = 0; k < n && k < 5; ++k) out_ms[k] = (float)(h->t_ms[k] / d);
   if (n > 5) out_ms[5] = (float)h->timed_steps;
+  if (n > 6) out_ms[6] = (float)(h->t_ms[5] / d);
   return DSMC_OK;
 }
 
@@ -1263,6 +1292,13 @@ uint32_t step_flags(const dsmc_solver* h, uint64_t stepno, bool* sample) {
 
 // Enqueue the kernels of one step reading buffer `par` (no host sync, no
 // memsets: safe to capture in a HIP graph).
+// Workgroups of the wall pass (each walks the queued tiles of its instance
+// with a stride of the grid): an eighth of the tiles, within [64, 4096] and
+// at most the tile count.
+uint32_t wall_grid(const InstDesc& I) {
+  return std::max(1u, std::min(I.ntiles, std::min(4096u, std::max(64u, I.ntiles / 8u))));
+}
+
 int enqueue_step(dsmc_solver* h, uint32_t par, uint32_t flags, hipEvent_t* ev) {
   InstDesc& I = h->h;
   hipStream_t st = h->stream;
@@ -1273,12 +1309,14 @@ int enqueue_step(dsmc_solver* h, uint32_t par, uint32_t flags, hipEvent_t* ev) {
   if (ev) HIPC(hipEventRecord(ev[2], st));
   dsmc_launch::tile(I.mode, h->d_desc, par, flags, I.n_vwg, st);
   if (ev) HIPC(hipEventRecord(ev[3], st));
+  if (I.near_tiles) dsmc_launch::walls(h->d_desc, par ^ 1u, flags, wall_grid(I), st);
+  if (ev) HIPC(hipEventRecord(ev[4], st));
   if (I.n_face_cells) {
     dsmc_launch::inject_count(h->d_desc, std::min(I.n_streams, I.n_face_cells), st);
     dsmc_launch::step_scan(h->d_desc, 1, 1, I.n_face_cells, h->scan_scratch, st);
     dsmc_launch::inject_generate(h->d_desc, par ^ 1u, std::min(I.n_streams, I.n_face_cells), st);
   }
-  if (ev) HIPC(hipEventRecord(ev[4], st));
+  if (ev) HIPC(hipEventRecord(ev[5], st));
   return DSMC_OK;
 }
 
@@ -1288,14 +1326,14 @@ int step_once(dsmc_solver* h) {
   uint32_t flags = step_flags(h, stepno, &sample);
   hipEvent_t* ev = nullptr;
   if (h->timing) {
-    if (h->ev_used + 5 > 5 * 4096) CHECK_RC(fold_timings(h));
-    while (h->ev_pool.size() < h->ev_used + 5) {
+    if (h->ev_used + 6 > 6 * 4096) CHECK_RC(fold_timings(h));
+    while (h->ev_pool.size() < h->ev_used + 6) {
       hipEvent_t e;
       HIPC(hipEventCreate(&e));
       h->ev_pool.push_back(e);
     }
     ev = &h->ev_pool[h->ev_used];
-    h->ev_used += 5;
+    h->ev_used += 6;
   }
   CHECK_RC(enqueue_step(h, h->par, flags, ev));
   HIPC(hipGetLastError());
@@ -1343,6 +1381,7 @@ struct dsmc_batch {
   InstDesc* d_descs = nullptr;
   uint32_t* scratch = nullptr;
   uint32_t n = 0, mode = 0, max_cells = 0, max_faces = 0, max_vwg = 0, max_inj_threads = 0, scatter_grid = 0;
+  uint32_t wall_grid = 0;
   bool use_graphs = true;
   std::map<uint64_t, hipGraphExec_t> graphs;
   ~dsmc_batch() {
@@ -1372,6 +1411,7 @@ int enqueue_batch_step(dsmc_batch* b, uint32_t par, uint32_t flags) {
   dsmc_launch::step_scan(b->d_descs, b->n, 0, b->max_cells, b->scratch, st);
   dsmc_launch::scatter(b->d_descs, par, b->scatter_grid, st, b->n);
   dsmc_launch::tile(b->mode, b->d_descs, par, flags, b->max_vwg, st, b->n);
+  if (b->wall_grid) dsmc_launch::walls(b->d_descs, par ^ 1u, flags, b->wall_grid, st, b->n);
   if (b->max_faces) {
     dsmc_launch::inject_count(b->d_descs, b->max_inj_threads, st, b->n);
     dsmc_launch::step_scan(b->d_descs, b->n, 1, b->max_faces, b->scratch, st);
@@ -1407,6 +1447,7 @@ int dsmc_batch_create(dsmc_solver* const* hs, uint32_t n, dsmc_batch** out) {
     b->max_vwg = std::max(b->max_vwg, h->h.n_vwg);
     b->max_inj_threads = std::max(b->max_inj_threads, std::min(h->h.n_streams, h->h.n_face_cells));
     b->scatter_grid = std::max(b->scatter_grid, h->scatter_grid);
+    if (h->h.near_tiles) b->wall_grid = std::max(b->wall_grid, wall_grid(h->h));
   }
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);

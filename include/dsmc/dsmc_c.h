@@ -211,6 +211,9 @@ int dsmc_set_tile_cells(dsmc_solver* h, uint32_t tile_cells);
  * 912-1000, seeded per cell as CreateRandomSeedsUint4 does at :1846-1886). */
 int dsmc_get_taus_state(dsmc_solver* h, uint32_t* out, uint32_t n_words);
 int dsmc_set_taus_state(dsmc_solver* h, const uint32_t* in, uint32_t n_words);
+/* PHYSICS with a body: 1 for the cells within one cell of the body's
+ * collision-grid cells (their particles go through the wall pass), else 0. */
+int dsmc_get_near_cells(dsmc_solver* h, uint8_t* out, uint32_t n_cells);
 /* Per-cell fluid volume fraction used by PHYSICS mode (1 = no body). */
 int dsmc_get_fluid_fraction(dsmc_solver* h, float* out, uint32_t n_cells);
 /* Collision grid of the geometry: flags (0 empty, 1 triangles, 2 filled
@@ -218,7 +221,8 @@ int dsmc_get_fluid_fraction(dsmc_solver* h, float* out, uint32_t n_cells);
 int dsmc_get_collision_grid(dsmc_solver* h, uint32_t* flags, uint32_t n);
 /* Average milliseconds per step of each kernel class {scan, scatter, cells,
  * inject, total} over the steps timed since dsmc_set_timing; out_ms[5]
- * (when n > 5) receives the number of timed steps. */
+ * (when n > 5) receives the number of timed steps and out_ms[6] (n > 6) the
+ * wall pass of PHYSICS runs with a body. */
 int dsmc_get_timings(dsmc_solver* h, float* out_ms, uint32_t n);
 /* Enable hipEvent timing of each kernel class (off by default). Events are
  * recorded on the solver's stream without host synchronisation and resolved

@@ -72,6 +72,8 @@ def lib() -> C.CDLL:
         "oracle_taus_draws": ([pU, I32, U32, pU], None),
         "oracle_taus_period": ([I32, U32, C.c_uint64], C.c_uint64),
         "oracle_mt_window_after": ([U32, U32, U32, U32, U32, pU], None),
+        "oracle_philox": ([pU, pU, pU], None),
+        "oracle_set_near_cells": ([P, C.POINTER(C.c_uint8)], I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -150,6 +152,15 @@ def taus_draws(seed, n: int, component: int = 0) -> np.ndarray:
     return out
 
 
+def philox(ctr, key) -> np.ndarray:
+    """Philox4x32-10 of a 4-word counter under a 2-word key."""
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().oracle_philox(up(c), up(k), up(out))
+    return out
+
+
 def mt_window_after(a: int, b: int, c: int, seed: int, d: int) -> np.ndarray:
     """Sliding-window state (19 words + a zero) of one stream after d draws."""
     out = np.zeros(20, np.uint32)
@@ -203,6 +214,11 @@ class OracleSim:
         lam = np.ascontiguousarray(d["inj_lambda"], np.float32)
         lib().oracle_set_scalars(o._o, float(d["fnum"]), float(d["vhs_k"]), float(d["vhs_exp"]), fp(lam))
         return o
+
+    def set_near_cells(self, flags) -> None:
+        f = np.ascontiguousarray(flags, np.uint8)
+        self._near = f
+        assert lib().oracle_set_near_cells(self._o, f.ctypes.data_as(C.POINTER(C.c_uint8))) == 0
 
     def set_geometry(self, mesh, grid: dict, fluid_frac=None) -> None:
         v = np.ascontiguousarray(mesh.vertices, np.float32)

@@ -36,10 +36,12 @@
 #endif
 
 #include "../cudadsmc_amd/csrc/common/detmath.h"
+#include "../cudadsmc_amd/csrc/common/philox.h"
 
 namespace {
 
 const uint32_t INV = 0xFFFFFFFFu;
+const uint32_t DEFER = 0xFFFFFFFEu;  // PHYSICS: particle left for the wall pass
 const float BOLZ_REF = 1.3806e-23f;
 const double BOLZ = 1.380649e-23;
 const float PI_REF = 3.14159265358979f;
@@ -172,6 +174,7 @@ struct oracle_sim {
   uint32_t cgdim[3];
   std::vector<uint32_t> cgflag, cgstart, cgend, cgtri;
   std::vector<float> frac;
+  std::vector<uint8_t> nearc;  // PHYSICS: cells whose particles go through the wall pass
   // RNG streams
   std::vector<MtBlock> rng;
   std::vector<TausState> taus;  // per cell, REFERENCE mode with mt = 0 (empty otherwise)
@@ -184,6 +187,7 @@ struct oracle_sim {
   std::vector<double> acc;   // [G][7]
   std::vector<uint32_t> perm, cell_start, hits;
   uint32_t n_store = 0;
+  uint32_t stepno = 0;
 
   // ---------------------------------------------------------- keys
   uint32_t cell_key(float px, float py, float pz) const {
@@ -379,66 +383,88 @@ struct oracle_sim {
     return true;
   }
 
-  uint32_t move_physics(float& px, float& py, float& pz, float& ux, float& uy, float& uz, MtBlock& r, uint32_t& nh,
-                        double* a) const {
+  // PHYSICS move, in two parts as the engine runs it: a particle whose
+  // free-flight segment box meets the collision grid's box is deferred to
+  // the wall pass (move_walls, after all tiles); all others just fly.
+  bool near_body(uint32_t c, float ux, float uy, float uz) const {
+    if (!has_geom) return false;
+    bool flagged = nearc.empty() || nearc[c] != 0;  // without the engine's flags: every cell
+    return flagged || std::fabs(ux) * dt >= cell[0] || std::fabs(uy) * dt >= cell[1] || std::fabs(uz) * dt >= cell[2];
+  }
+
+  uint32_t faces_key(float& px, float& py, float& pz, float& ux, float& uy, float& uz) const {
+    if (!face(px, ux, gmin[0], gmax[0], s.face_bc[0], s.face_bc[1])) return INV;
+    if (!face(py, uy, gmin[1], gmax[1], s.face_bc[2], s.face_bc[3])) return INV;
+    if (!face(pz, uz, gmin[2], gmax[2], s.face_bc[4], s.face_bc[5])) return INV;
+    return cell_key(px, py, pz);
+  }
+
+  uint32_t move_free(float& px, float& py, float& pz, float& ux, float& uy, float& uz) const {
+    px = px + ux * dt;
+    py = py + uy * dt;
+    pz = pz + uz * dt;
+    return faces_key(px, py, pz, ux, uy, uz);
+  }
+
+  // Walls, then faces. Diffuse re-emission of bounce b draws Philox4x32-10 at
+  // counter (slot, step, 0, b) under key (seed, 0x6A09E667).
+  uint32_t move_walls(float& px, float& py, float& pz, float& ux, float& uy, float& uz, uint32_t slot,
+                      uint32_t& nh, double* a) const {
     nh = 0;
     float tl = dt;
-    if (has_geom) {
-      uint32_t skip = INV;
-      for (int bnc = 0; bnc < 8; ++bnc) {
-        float sp = std::sqrt(ux * ux + uy * uy + uz * uz);
-        float dist = sp * tl;
-        if (!(dist > 0.0f)) break;
-        float dx = ux / sp, dy = uy / sp, dz = uz / sp;
-        float sh;
-        uint32_t tri;
-        if (!nearest_hit(px, py, pz, dx, dy, dz, dist, skip, sh, tri)) break;
-        const V3& n = tn[tri];
-        float hx = px + dx * sh, hy = py + dy * sh, hz = pz + dz * sh;
-        float wx, wy, wz;
-        if (vm_wall > 0.0f) {
-          float r1 = r.uniform(), r2 = r.uniform(), r3 = r.uniform();
-          float vn = vm_wall * std::sqrt(-dm_logf(r1));
-          float rt = vm_wall * std::sqrt(-dm_logf(r2));
-          float sn, cs;
-          dm_sincosf(6.28318530717958647692f * r3, &sn, &cs);
-          float a1 = rt * cs, a2 = rt * sn;
-          float t1[3], t2[3];
-          tangents(n.x, n.y, n.z, t1, t2);
-          wx = n.x * vn + t1[0] * a1 + t2[0] * a2;
-          wy = n.y * vn + t1[1] * a1 + t2[1] * a2;
-          wz = n.z * vn + t1[2] * a1 + t2[2] * a2;
-        } else {
-          float d = ux * n.x + uy * n.y + uz * n.z;
-          wx = ux - 2.0f * d * n.x;
-          wy = uy - 2.0f * d * n.y;
-          wz = uz - 2.0f * d * n.z;
-        }
-        double m = mass;
-        a[3] += m * ((double)ux - (double)wx);
-        a[4] += m * ((double)uy - (double)wy);
-        a[5] += m * ((double)uz - (double)wz);
-        a[6] += 0.5 * m * (((double)ux * ux + (double)uy * uy + (double)uz * uz) - ((double)wx * wx + (double)wy * wy + (double)wz * wz));
-        ++nh;
-        tl = tl - sh / sp;
-        px = hx + n.x * 1.0e-6f;
-        py = hy + n.y * 1.0e-6f;
-        pz = hz + n.z * 1.0e-6f;
-        ux = wx;
-        uy = wy;
-        uz = wz;
-        skip = tri;
+    uint32_t skip = INV;
+    for (int bnc = 0; bnc < 8; ++bnc) {
+      float sp = std::sqrt(ux * ux + uy * uy + uz * uz);
+      float dist = sp * tl;
+      if (!(dist > 0.0f)) break;
+      float dx = ux / sp, dy = uy / sp, dz = uz / sp;
+      float sh;
+      uint32_t tri;
+      if (!nearest_hit(px, py, pz, dx, dy, dz, dist, skip, sh, tri)) break;
+      const V3& n = tn[tri];
+      float hx = px + dx * sh, hy = py + dy * sh, hz = pz + dz * sh;
+      float wx, wy, wz;
+      if (vm_wall > 0.0f) {
+        dsmc_rng::U32x4 c = {{slot, stepno, 0u, (uint32_t)bnc}};
+        dsmc_rng::U32x4 rr = dsmc_rng::philox4x32_10(c, (uint32_t)s.seed, 0x6A09E667u);
+        float r1 = dsmc_rng::px_unit(rr.v[0]), r2 = dsmc_rng::px_unit(rr.v[1]), r3 = dsmc_rng::px_unit(rr.v[2]);
+        float vn = vm_wall * std::sqrt(-dm_logf(r1));
+        float rt = vm_wall * std::sqrt(-dm_logf(r2));
+        float sn, cs;
+        dm_sincosf(6.28318530717958647692f * r3, &sn, &cs);
+        float a1 = rt * cs, a2 = rt * sn;
+        float t1[3], t2[3];
+        tangents(n.x, n.y, n.z, t1, t2);
+        wx = n.x * vn + t1[0] * a1 + t2[0] * a2;
+        wy = n.y * vn + t1[1] * a1 + t2[1] * a2;
+        wz = n.z * vn + t1[2] * a1 + t2[2] * a2;
+      } else {
+        float d = ux * n.x + uy * n.y + uz * n.z;
+        wx = ux - 2.0f * d * n.x;
+        wy = uy - 2.0f * d * n.y;
+        wz = uz - 2.0f * d * n.z;
       }
+      double m = mass;
+      a[3] += m * ((double)ux - (double)wx);
+      a[4] += m * ((double)uy - (double)wy);
+      a[5] += m * ((double)uz - (double)wz);
+      a[6] += 0.5 * m * (((double)ux * ux + (double)uy * uy + (double)uz * uz) - ((double)wx * wx + (double)wy * wy + (double)wz * wz));
+      ++nh;
+      tl = tl - sh / sp;
+      px = hx + n.x * 1.0e-6f;
+      py = hy + n.y * 1.0e-6f;
+      pz = hz + n.z * 1.0e-6f;
+      ux = wx;
+      uy = wy;
+      uz = wz;
+      skip = tri;
     }
     if (tl > 0.0f) {
       px = px + ux * tl;
       py = py + uy * tl;
       pz = pz + uz * tl;
     }
-    if (!face(px, ux, gmin[0], gmax[0], s.face_bc[0], s.face_bc[1])) return INV;
-    if (!face(py, uy, gmin[1], gmax[1], s.face_bc[2], s.face_bc[3])) return INV;
-    if (!face(pz, uz, gmin[2], gmax[2], s.face_bc[4], s.face_bc[5])) return INV;
-    return cell_key(px, py, pz);
+    return faces_key(px, py, pz, ux, uy, uz);
   }
 
   // ------------------------------------------------------------- NTC
@@ -551,6 +577,7 @@ struct oracle_sim {
 
   // ------------------------------------------------------------- step
   void step(bool sample) {
+    ++stepno;  // the engine's step counter (counter of the wall draws)
     // stable counting sort of the storage by key (the sort_by_key + bounds)
     std::vector<uint32_t> cnt(C, 0);
     for (uint32_t i = 0; i < n_store; ++i)
@@ -596,11 +623,15 @@ struct oracle_sim {
           }
         }
         // move, one stream per position modulo the thread count
+        uint32_t cq = c0;
         for (uint32_t q = p0; q < p1; ++q) {
+          while (q >= cell_start[cq + 1]) ++cq;
           MtBlock& r = rng[(size_t)vw * TT + (q - p0) % TT];
           uint32_t nh = 0, k;
           if (phys) {
-            k = move_physics(nx[q], ny[q], nz[q], nvx[q], nvy[q], nvz[q], r, nh, a);
+            (void)r;
+            k = near_body(cq, nvx[q], nvy[q], nvz[q]) ? DEFER
+                                                      : move_free(nx[q], ny[q], nz[q], nvx[q], nvy[q], nvz[q]);
           } else {
             nh = move_reference(nx[q], ny[q], nz[q], nvx[q], nvy[q], nvz[q], dt);
             k = cell_key(nx[q], ny[q], nz[q]);
@@ -634,6 +665,25 @@ struct oracle_sim {
           }
         }
       }
+    }
+    // wall pass over the deferred particles, in slot order
+    if (phys && has_geom) {
+      double wf[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma omp parallel
+      {
+        double a7[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma omp for schedule(dynamic, 4096)
+        for (int64_t qq = 0; qq < (int64_t)N; ++qq) {
+          const uint32_t q = (uint32_t)qq;
+          if (nkey[q] != DEFER) continue;
+          uint32_t nh = 0;
+          nkey[q] = move_walls(nx[q], ny[q], nz[q], nvx[q], nvy[q], nvz[q], q, nh, a7);
+          hits[q] = nh;
+        }
+#pragma omp critical
+        for (int k = 0; k < 4; ++k) wf[k] += a7[3 + k];
+      }
+      for (int k = 0; k < 4; ++k) acc[3 + k] += wf[k];
     }
     x.swap(nx); y.swap(ny); z.swap(nz);
     vx.swap(nvx); vy.swap(nvy); vz.swap(nvz);
@@ -944,6 +994,13 @@ int oracle_set_geometry(oracle_sim* o, const float* verts, uint32_t nv, const ui
   return 0;
 }
 
+int oracle_set_near_cells(oracle_sim* o, const uint8_t* flags) {
+  if (!o) return -1;
+  if (flags) o->nearc.assign(flags, flags + o->C);
+  else o->nearc.clear();
+  return 0;
+}
+
 int oracle_set_scalars(oracle_sim* o, float fnum, float vhs_k, float vhs_exp, const float inj_lambda[6]) {
   if (!o) return -1;
   o->fnum = fnum;
@@ -996,6 +1053,13 @@ void oracle_mt_window_after(uint32_t a, uint32_t b, uint32_t c, uint32_t seed, u
   for (uint32_t k = 0; k < d; ++k) m.next();
   m.to_window(out);
   out[19] = 0u;
+}
+
+// One Philox4x32-10 block (the wall re-emission generator).
+void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  dsmc_rng::U32x4 c = {{ctr[0], ctr[1], ctr[2], ctr[3]}};
+  dsmc_rng::U32x4 r = dsmc_rng::philox4x32_10(c, key[0], key[1]);
+  for (int k = 0; k < 4; ++k) out[k] = r.v[k];
 }
 
 // Draws of one Hybrid-Taus stream (the combined generator, or with

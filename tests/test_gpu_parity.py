@@ -150,6 +150,8 @@ def _mirror(sol: dm.Solver, mesh=None) -> oracle.OracleSim:
     if mesh is not None:
         ff = sol.fluid_fraction() if sol.settings.mode == dm.MODE_PHYSICS else None
         o.set_geometry(mesh, sol.collision_grid(), ff)
+        if sol.settings.mode == dm.MODE_PHYSICS:
+            o.set_near_cells(sol.near_cells())
     o.set_rng(sol.rng_state())
     o.set_ntc(sol.ntc_state())
     return o
@@ -239,6 +241,10 @@ def test_gpu_physics_steps_bit_exact():
     coll = sol.collision_stats()
     assert np.allclose(coll[:3], acc[:3], rtol=1e-12)
     assert coll[1] > 0 and total_hits > 0
+    # wall force of the wall pass: same per-hit terms, summed in another order
+    F = sol.drag()
+    want = acc[3:6] * float(sol.derived()["fnum"]) / (10 * s.dt)
+    assert np.allclose(F[:3], want, rtol=1e-9, atol=1e-12 * np.abs(want).max()), (F, want)
 
 
 def test_gpu_overflow_tiles_bit_exact():

@@ -95,6 +95,10 @@ def test_free_molecular_sphere_drag():
     note(f"C_D {cd:.4f} free-molecular {cd_fm:.4f}")
     assert abs(cd / cd_fm - 1) < 0.04
     assert abs(F[1]) < 0.05 * abs(F[0]) and abs(F[2]) < 0.05 * abs(F[0])
+    # no particle got through the wall: none inside the inscribed sphere of the mesh
+    p = sol.particles()["pos"].astype(np.float64)
+    rr = np.sqrt(((p - np.array([[0.08], [0.05], [0.05]])) ** 2).sum(0))
+    assert int((rr < 0.99 * r).sum()) == 0
 
 
 def test_freestream_preserved_by_open_faces():

@@ -163,3 +163,15 @@ def test_mt_window_export_matches_the_recurrence():
         w = oracle.mt_window_after(a, 0x9D2C5680, 0xEFC60000, seed, d)
         assert w[:19].tolist() == x[d:d + 19], d
         assert w[19] == 0
+
+
+@pytest.mark.parametrize("ctr,key,want", [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+])
+def test_philox4x32_10_known_answers(ctr, key, want):
+    """Known-answer vectors of Philox4x32-10 (Salmon et al., SC'11; the
+    published test vectors: zeros, all ones, and the digits of pi)."""
+    assert oracle.philox(ctr, key).tolist() == list(want)
