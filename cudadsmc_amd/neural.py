@@ -11,7 +11,7 @@ Architecture (about 2.1M parameters at width 96):
 * conv 3x3x3 32 -> W, ReLU;
 * residual blocks x <- x + ReLU(conv 3x3x3 W -> W, dilation d_i);
 * 1x1 head W -> 32, of which the first five channels are the fields
-  (n/n_inf, u/U, v/U, w/U, T/T_inf).
+  (log n/n_inf, u/U, v/U, w/U, log T/T_inf).
 
 Training minimises the masked mean-squared error with Adam on FP32 master
 weights. Every tensor operation of the forward and backward passes is a
@@ -122,16 +122,21 @@ def normalise_params(U, n, T_inf, T_w, r) -> np.ndarray:
 
 
 def fields_to_targets(n, u, v, w, T, n_inf, U, T_inf) -> np.ndarray:
-    """Fields of one case ([Z][Y][X] each) -> the five operator outputs,
-    each scaled by its freestream value."""
-    return np.stack([np.asarray(n) / n_inf, np.asarray(u) / U, np.asarray(v) / U, np.asarray(w) / U,
-                     np.asarray(T) / T_inf], axis=-1).astype(np.float32)
+    """Fields of one case ([Z][Y][X] each) -> the five operator outputs:
+    log(n/n_inf), u/U, v/U, w/U, log(T/T_inf). Cells without samples (n or
+    T not positive) get 0 and must be masked out of the loss."""
+    n = np.asarray(n, np.float64)
+    T = np.asarray(T, np.float64)
+    ok_n, ok_T = n > 0, T > 0
+    ln = np.where(ok_n, np.log(np.where(ok_n, n, 1.0) / n_inf), 0.0)
+    lt = np.where(ok_T, np.log(np.where(ok_T, T, 1.0) / T_inf), 0.0)
+    return np.stack([ln, np.asarray(u) / U, np.asarray(v) / U, np.asarray(w) / U, lt], axis=-1).astype(np.float32)
 
 
 def targets_to_fields(t, n_inf, U, T_inf):
     """Inverse of fields_to_targets: returns n, u, v, w, T."""
     t = np.asarray(t, np.float64)
-    return (n_inf * t[..., 0], U * t[..., 1], U * t[..., 2], U * t[..., 3], T_inf * t[..., 4])
+    return (n_inf * np.exp(t[..., 0]), U * t[..., 1], U * t[..., 2], U * t[..., 3], T_inf * np.exp(t[..., 4]))
 
 
 # ------------------------------------------------------------------ network

@@ -106,8 +106,9 @@ def train(out: Path, iters: int, batch: int, lr: float, width: int, seed: int, h
         a0 = a_all[idx].reshape(batch * M1, neural.C_IN).contiguous()
         tgt = dev_t[idx].reshape(batch * M1, 5).contiguous()
         msk = dev_m[idx].reshape(batch * M1).contiguous()
-        # cosine decay from lr to lr/20
-        lr_t = lr * (0.05 + 0.95 * 0.5 * (1 + math.cos(math.pi * it / max(1, iters))))
+        # linear warm-up over 100 iterations, then cosine decay to lr/20
+        warm = min(1.0, (it + 1) / 100.0)
+        lr_t = warm * lr * (0.05 + 0.95 * 0.5 * (1 + math.cos(math.pi * it / max(1, iters))))
         loss = op.train_step(a0, tgt, msk, batch, lr_t)
         if it % 50 == 0 or it == iters - 1:
             lv = float(loss.item())
@@ -132,6 +133,7 @@ def predicted_fields(op: neural.FieldOperator, cases: list[ens.Case]) -> list[di
         n, u, v, w, T = neural.targets_to_fields(t, c.n, c.U, c.T_inf)
         ok = all(np.all(np.isfinite(a)) for a in (n, u, v, w, T))
         ok = ok and float(np.max(n)) < 100.0 * c.n and float(np.max(T)) < 100.0 * c.T_inf
+        ok = ok and float(np.min(T)) > 0.01 * c.T_inf
         if not ok:
             out.append(None)
             continue
@@ -224,7 +226,8 @@ def time_to_steady(cases, refs, fields, chunk: int, max_steps: int, tol: float, 
     for st, s in zip(state, sols):
         m, e = st["drag"].cd() if st["drag"] else (float("nan"), float("nan"))
         rows.append({"steady_step": st["steady_step"], "steady_wall_s": st.get("steady_wall_s"),
-                     "sampling_from": st["sampling_from"], "final_err": st["err"], "cd": m, "cd_sigma": e})
+                     "sampling_from": st["sampling_from"], "final_err": st["err"], "cd": m, "cd_sigma": e,
+                     "n_history": st["n_hist"]})
         s.close()
     done = all(r["steady_step"] is not None for r in rows)
     # the ensemble is steady when its last member is
@@ -279,24 +282,24 @@ def main(argv=None) -> None:
     ap.add_argument("--out", default="gpurun_out/warmstart")
     ap.add_argument("--runs", type=int, default=512)
     ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--transient", type=int, default=1500)
-    ap.add_argument("--sampling", type=int, default=1500)
+    ap.add_argument("--transient", type=int, default=2000)
+    ap.add_argument("--sampling", type=int, default=2000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--nmin", type=float, default=1e4)
     ap.add_argument("--nmax", type=float, default=5e5)
-    ap.add_argument("--iters", type=int, default=3000)
+    ap.add_argument("--iters", type=int, default=6000)
     ap.add_argument("--train-batch", type=int, default=8)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--width", type=int, default=96)
     ap.add_argument("--cases", type=int, default=32)
     ap.add_argument("--eval-seed", type=int, default=99)
     ap.add_argument("--eval-n", type=int, default=500000)
     ap.add_argument("--ref-transient", type=int, default=2000)
-    ap.add_argument("--ref-sampling", type=int, default=6000)
+    ap.add_argument("--ref-sampling", type=int, default=8000)
     ap.add_argument("--chunk", type=int, default=50)
-    ap.add_argument("--max-steps", type=int, default=6000)
+    ap.add_argument("--max-steps", type=int, default=4000)
     ap.add_argument("--tol", type=float, default=0.02)
-    ap.add_argument("--block", type=int, default=2)
+    ap.add_argument("--block", type=int, default=4)
     ap.add_argument("--settle", type=float, default=0.005)
     a = ap.parse_args(argv)
     out = Path(a.out)

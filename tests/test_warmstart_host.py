@@ -44,7 +44,10 @@ def test_targets_round_trip():
     t = neural.fields_to_targets(n, u, v, w, T, 2e20, 1800.0, 220.0)
     back = neural.targets_to_fields(t, 2e20, 1800.0, 220.0)
     for a, b in zip((n, u, v, w, T), back):
-        np.testing.assert_allclose(b, a, rtol=2e-6, atol=1e-3 * np.max(np.abs(a)) * 1e-3)
+        np.testing.assert_allclose(b, a, rtol=2e-6, atol=1e-6 * np.max(np.abs(a)))
+    # cells without samples map to 0 (masked in training)
+    z = neural.fields_to_targets(np.zeros(3), np.zeros(3), np.zeros(3), np.zeros(3), np.zeros(3), 1e20, 1000.0, 200.0)
+    assert np.all(z == 0.0)
 
 
 def test_params_normalisation_is_order_one():
