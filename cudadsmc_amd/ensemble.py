@@ -149,10 +149,12 @@ class DragWindows:
 
 
 def run_batch(cases: list[Case], transient: int, sampling: int, windows: int = 20, dt: float = 1.0e-6,
-              n_streams: int | None = None, fields: list[dict] | None = None, graphs: bool = True) -> dict:
+              n_streams: int | None = None, fields: list[dict] | None = None, graphs: bool = True,
+              keep_stats: bool = False) -> dict:
     """Run the cases as one batch: transient steps, then sampling steps in
     `windows` equal windows. Returns per-case fields, drag C_d (mean,
-    batch-means sigma), particle counts and timings."""
+    batch-means sigma), particle counts and timings; with `keep_stats` also
+    the per-cell moment sums (C, 5) of the sampling window."""
     t0 = time.perf_counter()
     sols = [make_solver(c, dt, n_streams, None if fields is None else fields[i]) for i, c in enumerate(cases)]
     t_setup = time.perf_counter() - t0
@@ -188,7 +190,10 @@ def run_batch(cases: list[Case], transient: int, sampling: int, windows: int = 2
     for d, s in zip(drags, sols):
         f = s.fields()
         out["fields"].append(f)
-        out["counts"].append(s.cell_stats()[:, 0].astype(np.float32))
+        cs = s.cell_stats()
+        out["counts"].append(cs[:, 0].astype(np.float32))
+        if keep_stats:
+            out.setdefault("stats", []).append(cs)
         m, e = d.cd()
         out["cd"].append(m)
         out["cd_sigma"].append(e)

@@ -6,19 +6,21 @@ the warm-versus-cold time-to-steady evaluation.
     python -m cudadsmc_amd.warmstart eval  --out DIR [--cases 32 ...]
 
 Time-to-steady of one arm (cold: uniform freestream loading; warm: local
-Maxwellians from the operator's predicted fields) is measured in steps of
-`chunk`:
+Maxwellians from the operator's predicted fields) is measured in chunks of
+`chunk` steps:
 
-* start rule (both arms): sampling starts at the first chunk boundary where
-  the simulated particle count changed by less than `settle` (relative) over
-  each of the last two chunks;
-* steady: the accumulated fields n, |u| and T, averaged over blocks of
-  `block`^3 cells, are within `tol` relative L2 of a long cold reference
-  run of the same case, and the sampled C_d is within 3 sigma (batch means,
-  combined with the reference's) of the reference C_d.
+* start rule (both arms, no reference needed): SteadyDetector compares the
+  block fields of consecutive chunks with those of the two halves of a
+  chunk; sampling restarts from zero at the end of the first chunk whose
+  remaining drift is within its noise;
+* steady: the fields accumulated since then, averaged over blocks of
+  `block`^3 cells, are within `tol` relative L2 (n, |u| and T) of a long
+  cold reference run of the same case, and the sampled C_d is within
+  3 sigma (batch means, combined with the reference's) of the reference C_d.
 
 The warm arm falls back to a cold start when the prediction is not finite or
-not physical. Wall time counts the operator's inference and the loading.
+not physical. Wall time counts the operator's inference, the loading and the
+read-outs of both rules.
 """
 from __future__ import annotations
 
@@ -168,51 +170,155 @@ def field_error(f: dict, ref: dict, frac: np.ndarray, U: float, block: int) -> f
     return float(max(errs))
 
 
+def block_moments(stats: np.ndarray, block: int) -> np.ndarray:
+    """Per-cell moment sums (C, 5: count, sum v_x, v_y, v_z, sum |v|^2;
+    X fastest) -> per-block sums (n_blocks, 5)."""
+    X, Y, Z = ens.GRID
+    a = np.asarray(stats, np.float64).reshape(Z // block, block, Y // block, block, X // block, block, 5)
+    return a.sum(axis=(1, 3, 5)).reshape(-1, 5)
+
+
+def moment_fields(m: np.ndarray, steps: int) -> np.ndarray:
+    """Block moment sums over `steps` sampled steps -> (n_blocks, 3): particles
+    per step, bulk speed, and the thermal part of <|v|^2> (3kT/m)."""
+    N = m[:, 0]
+    safe = np.maximum(N, 1.0)
+    u = m[:, 1:4] / safe[:, None]
+    th = m[:, 4] / safe - (u * u).sum(1)
+    return np.stack([N / max(steps, 1), np.linalg.norm(u, axis=1), th], 1)
+
+
+def moment_change(a: np.ndarray, b: np.ndarray, mask: np.ndarray) -> np.ndarray:
+    """Per field (density, speed, temperature): relative L2 difference of
+    block fields a and b over the masked blocks."""
+    a, b = a[mask], b[mask]
+    return np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(b, axis=0), 1e-30)
+
+
+def block_mask(frac: np.ndarray, block: int) -> np.ndarray:
+    """Blocks whose cells are all fluid (flattened like block_moments)."""
+    X, Y, Z = ens.GRID
+    return (block_mean(np.asarray(frac, np.float64).reshape(Z, Y, X), block) > 0.999).ravel()
+
+
+class SteadyDetector:
+    """Sampling-start rule that needs no reference and treats warm and cold
+    starts alike. The run is sampled in windows of `half` steps; two windows
+    make a chunk. With block fields F, noise sigma per chunk and a systematic
+    drift D per chunk,
+
+        |F(chunk j) - F(chunk j-1)|^2            ~ 2 sigma^2 + D^2
+        |F(2nd half of j) - F(1st half of j)|^2  ~ 4 sigma^2 + D^2 / 4,
+
+    so the ratio R = first / second is 1/2 for independent noise and tends
+    to 4 while the fields drift. Fluctuations of a block persist for about
+    the time particles take to cross it (tens of steps), which makes
+    adjacent windows correlated and raises the stationary R towards 1. The
+    test passes when R averaged over the three fields (density, speed,
+    temperature) is at most 1: D below about the noise of one chunk.
+    Sampling starts at the end of that chunk.
+    """
+
+    def __init__(self, mask: np.ndarray, block: int, half: int):
+        self.mask, self.block, self.half = mask, block, half
+        self.prev_cum = None
+        self.halves: list[np.ndarray] = []
+        self.prev_chunk = None
+        self.last = None  # (chunk change, half change) of the last test
+
+    def feed(self, cum_stats: np.ndarray) -> bool:
+        """Cumulative per-cell sums at the end of a half window -> True when
+        the chunk that just ended passes the test."""
+        cum = block_moments(cum_stats, self.block)
+        win = cum if self.prev_cum is None else cum - self.prev_cum
+        self.prev_cum = cum
+        self.halves.append(win)
+        if len(self.halves) < 2:
+            return False
+        A, B = self.halves
+        self.halves = []
+        chunk = A + B
+        fire = False
+        if self.prev_chunk is not None:
+            fc = moment_fields(chunk, 2 * self.half)
+            d_c = moment_change(fc, moment_fields(self.prev_chunk, 2 * self.half), self.mask)
+            d_h = moment_change(moment_fields(B, self.half), moment_fields(A, self.half), self.mask)
+            self.last = (d_c, d_h)
+            fire = bool(np.mean(d_c ** 2 / np.maximum(d_h ** 2, 1e-30)) <= 1.0)
+        self.prev_chunk = chunk
+        return fire
+
+
+def freestream_fields(c: ens.Case) -> dict:
+    C_ = ens.GRID[0] * ens.GRID[1] * ens.GRID[2]
+    one = np.ones(C_, np.float32)
+    return {"n": c.n * one, "u": c.U * one, "v": 0 * one, "w": 0 * one, "T": c.T_inf * one}
+
+
 def reference_runs(cases, transient, sampling, windows=20, batch=32) -> list[dict]:
     refs = []
     for b0 in range(0, len(cases), batch):
-        r = ens.run_batch(cases[b0:b0 + batch], transient, sampling, windows=windows)
+        r = ens.run_batch(cases[b0:b0 + batch], transient, sampling, windows=windows, keep_stats=True)
         for i in range(len(r["fields"])):
             refs.append({"fields": r["fields"][i], "cd": r["cd"][i], "cd_sigma": r["cd_sigma"][i],
-                         "frac": r["fluid_fraction"][i]})
+                         "frac": r["fluid_fraction"][i], "stats": r["stats"][i], "steps": sampling})
         print(f"[eval] reference {b0 + len(r['fields'])}/{len(cases)}", flush=True)
     return refs
 
 
-def time_to_steady(cases, refs, fields, chunk: int, max_steps: int, tol: float, block: int, settle: float,
-                   window: int) -> list[dict]:
-    """Advance all cases as one batch in chunks and record, per case, the step
-    and wall time at which its accumulated fields meet the steady criterion."""
+def time_to_steady(cases, refs, fields, chunk: int, max_steps: int, tol: float, block: int) -> dict:
+    """Advance all cases as one batch and record, per case, the step and wall
+    time at which the fields accumulated since the detector's sampling start
+    meet the steady criterion (block fields within `tol` of the reference and
+    C_d within 3 sigma of it, with at least three drag windows of `chunk`/2
+    steps). Also records, per chunk before the sampling start, the
+    instantaneous block-field error (density, speed, temperature) against
+    the reference (diagnostic, not used by either rule)."""
+    half = chunk // 2
+    assert half * 2 == chunk and half > 0
     t0 = time.perf_counter()
     sols = [ens.make_solver(c, fields=None if fields is None else fields[i]) for i, c in enumerate(cases)]
     load_s = time.perf_counter() - t0
     b = Batch(sols)
+    b.set_sampling(True)  # from step 0: the detector reads differences of the accumulators
     b.synchronize()
     k = len(cases)
-    state = [{"sampling_from": None, "steady_step": None, "n_hist": [sols[i].counts()[1]],
-              "drag": None, "err": None} for i in range(k)]
+    masks = [block_mask(r["frac"], block) for r in refs]
+    ref_bf = [moment_fields(block_moments(r["stats"], block), r["steps"]) for r in refs]
+    det = [SteadyDetector(masks[i], block, half) for i in range(k)]
+    state = [{"sampling_from": None, "steady_step": None, "drag": None, "err": None, "trace": [], "tests": [],
+              "cum0": None, "chunk_cum": None} for _ in range(k)]
     wall = 0.0
     step = 0
     while step < max_steps and any(st["steady_step"] is None for st in state):
         ts = time.perf_counter()
-        b.run(chunk)
+        b.run(half)
         b.synchronize()
-        wall += time.perf_counter() - ts
-        step += chunk
+        step += half
+        chunk_end = step % chunk == 0
         for i, (c, s, st) in enumerate(zip(cases, sols, state)):
             if st["steady_step"] is not None:
                 continue
             if st["sampling_from"] is None:
-                st["n_hist"].append(s.counts()[1])
-                h = st["n_hist"]
-                if len(h) >= 3 and all(abs(h[-j] - h[-j - 1]) <= settle * h[-j] for j in (1, 2)):
+                cum = s.cell_stats()
+                fire = det[i].feed(cum)
+                if chunk_end:
+                    bm = block_moments(cum, block)
+                    prev = st["chunk_cum"]
+                    win = bm if prev is None else bm - prev
+                    st["chunk_cum"] = bm
+                    st["trace"].append(moment_change(moment_fields(win, chunk), ref_bf[i], masks[i]).tolist())
+                    if det[i].last is not None:
+                        st["tests"].append([det[i].last[0].tolist(), det[i].last[1].tolist()])
+                if fire:
                     st["sampling_from"] = step
                     st["drag"] = ens.DragWindows(c, 1e-6)
-                    s.set_sampling(True)
+                    s.set_sampling(True, reset=True)
                 continue
             sampled = step - st["sampling_from"]
-            if sampled % window == 0:
-                st["drag"].update(s, sampled)
+            st["drag"].update(s, sampled)
+            if not chunk_end:
+                continue
             err = field_error(s.fields(), refs[i]["fields"], refs[i]["frac"], c.U, block)
             st["err"] = err
             m, e = st["drag"].cd()
@@ -220,14 +326,15 @@ def time_to_steady(cases, refs, fields, chunk: int, max_steps: int, tol: float, 
             ok_cd = len(st["drag"].means) >= 3 and abs(m - refs[i]["cd"]) <= 3.0 * sig
             if err <= tol and ok_cd:
                 st["steady_step"] = step
-                st["steady_wall_s"] = wall
+                st["steady_wall_s"] = wall + (time.perf_counter() - ts)
+        wall += time.perf_counter() - ts
     b.close()
     rows = []
     for st, s in zip(state, sols):
         m, e = st["drag"].cd() if st["drag"] else (float("nan"), float("nan"))
         rows.append({"steady_step": st["steady_step"], "steady_wall_s": st.get("steady_wall_s"),
                      "sampling_from": st["sampling_from"], "final_err": st["err"], "cd": m, "cd_sigma": e,
-                     "n_history": st["n_hist"]})
+                     "chunk_err": st["trace"], "detector_tests": st["tests"]})
         s.close()
     done = all(r["steady_step"] is not None for r in rows)
     # the ensemble is steady when its last member is
@@ -236,7 +343,7 @@ def time_to_steady(cases, refs, fields, chunk: int, max_steps: int, tol: float, 
 
 
 def evaluate(out: Path, n_cases: int, seed: int, N: int, ref_transient: int, ref_sampling: int, chunk: int,
-             max_steps: int, tol: float, block: int, settle: float) -> dict:
+             max_steps: int, tol: float, block: int, tag: str = "") -> dict:
     cases = ens.draw_cases(n_cases, seed, N_range=(N, N), seed0=seed * 1000)
     op = neural.FieldOperator.load(out / "operator.npz")
     t0 = time.perf_counter()
@@ -247,31 +354,42 @@ def evaluate(out: Path, n_cases: int, seed: int, N: int, ref_transient: int, ref
     infer_s = time.perf_counter() - t0
     fallback = sum(1 for w in warm if w is None)
     refs = reference_runs(cases, ref_transient, ref_sampling)
-    res_c = time_to_steady(cases, refs, None, chunk, max_steps, tol, block, settle, chunk)
+    res_c = time_to_steady(cases, refs, None, chunk, max_steps, tol, block)
     print(f"[eval] cold arm: {res_c['steps']} steps, all steady: {res_c['all_steady']}", flush=True)
-    res_w = time_to_steady(cases, refs, warm, chunk, max_steps, tol, block, settle, chunk)
+    res_w = time_to_steady(cases, refs, warm, chunk, max_steps, tol, block)
     print(f"[eval] warm arm: {res_w['steps']} steps, all steady: {res_w['all_steady']}", flush=True)
     rows = []
-    for c, rc, rw in zip(cases, res_c["rows"], res_w["rows"]):
-        rows.append({"case": asdict(c), "cold": rc, "warm": rw})
+    for c, ref, w, rc, rw in zip(cases, refs, warm, res_c["rows"], res_w["rows"]):
+        # error of the loaded fields themselves, before any step
+        e_pred = None if w is None else field_error(w, ref["fields"], ref["frac"], c.U, block)
+        e_free = field_error(freestream_fields(c), ref["fields"], ref["frac"], c.U, block)
+        rows.append({"case": asdict(c), "pred_err": e_pred, "freestream_err": e_free, "cold": rc, "warm": rw})
     sc = [r["cold"]["steady_step"] for r in rows]
     sw = [r["warm"]["steady_step"] for r in rows]
     both = [(a, b) for a, b in zip(sc, sw) if a is not None and b is not None]
+    med = lambda xs: float(np.median(xs)) if xs else None  # noqa: E731
     wc = res_c["wall_to_all_steady_s"]
     ww = None if res_w["wall_to_all_steady_s"] is None else res_w["wall_to_all_steady_s"] + infer_s
     summary = {"cases": n_cases, "N": N, "tol": tol, "block": block, "chunk": chunk, "max_steps": max_steps,
+               "ref_steps": [ref_transient, ref_sampling],
                "fallback_to_cold": fallback, "infer_s": infer_s,
+               "median_pred_err": med([r["pred_err"] for r in rows if r["pred_err"] is not None]),
+               "median_freestream_err": med([r["freestream_err"] for r in rows]),
                "cold_reached": sum(a is not None for a in sc), "warm_reached": sum(b is not None for b in sw),
-               "median_steps_cold": float(np.median([a for a, _ in both])) if both else None,
-               "median_steps_warm": float(np.median([b for _, b in both])) if both else None,
-               "steps_ratio_median": float(np.median([a / b for a, b in both])) if both else None,
+               "median_start_cold": med([r["cold"]["sampling_from"] for r in rows
+                                         if r["cold"]["sampling_from"] is not None]),
+               "median_start_warm": med([r["warm"]["sampling_from"] for r in rows
+                                         if r["warm"]["sampling_from"] is not None]),
+               "median_steps_cold": med([a for a, _ in both]),
+               "median_steps_warm": med([b for _, b in both]),
+               "steps_ratio_median": med([a / b for a, b in both]),
                "steps_all_cold": max(sc) if all(a is not None for a in sc) else None,
                "steps_all_warm": max(sw) if all(b is not None for b in sw) else None,
                "wall_to_all_steady_cold_s": wc, "wall_to_all_steady_warm_s": ww,
                "wall_ratio": (wc / ww) if (wc and ww) else None,
                "load_cold_s": res_c["load_s"], "load_warm_s": res_w["load_s"]}
     res = {"summary": summary, "rows": rows}
-    (out / "eval.json").write_text(json.dumps(res, indent=1, default=float))
+    (out / f"eval{tag}.json").write_text(json.dumps(res, indent=1, default=float))
     print(json.dumps(summary), flush=True)
     return res
 
@@ -296,11 +414,11 @@ def main(argv=None) -> None:
     ap.add_argument("--eval-n", type=int, default=500000)
     ap.add_argument("--ref-transient", type=int, default=2000)
     ap.add_argument("--ref-sampling", type=int, default=8000)
-    ap.add_argument("--chunk", type=int, default=50)
+    ap.add_argument("--chunk", type=int, default=40)
     ap.add_argument("--max-steps", type=int, default=4000)
     ap.add_argument("--tol", type=float, default=0.02)
     ap.add_argument("--block", type=int, default=4)
-    ap.add_argument("--settle", type=float, default=0.005)
+    ap.add_argument("--tag", default="")
     a = ap.parse_args(argv)
     out = Path(a.out)
     out.mkdir(parents=True, exist_ok=True)
@@ -310,7 +428,7 @@ def main(argv=None) -> None:
         train(out, a.iters, a.train_batch, a.lr, a.width, a.seed)
     if a.stage in ("eval", "all"):
         evaluate(out, a.cases, a.eval_seed, a.eval_n, a.ref_transient, a.ref_sampling, a.chunk, a.max_steps, a.tol,
-                 a.block, a.settle)
+                 a.block, a.tag)
 
 
 if __name__ == "__main__":

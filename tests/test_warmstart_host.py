@@ -105,3 +105,93 @@ def test_drag_windows_batch_means():
     m, e = d.cd()
     np.testing.assert_allclose(d.means, [1.0, 2.0, 3.0], rtol=1e-12)
     assert abs(m - 2.0) < 1e-12 and abs(e - 1.0 / math.sqrt(3)) < 1e-12
+
+
+def _half_window(rng, lam, u, th, steps):
+    """Per-cell moment sums of `steps` sampled steps for particles with bulk
+    velocity u (C, 3) and thermal spread th (3kT/m): count, sum v, sum |v|^2."""
+    C_ = len(lam)
+    n = rng.poisson(lam * steps).astype(np.float64)
+    s1 = n[:, None] * u + rng.normal(size=(C_, 3)) * np.sqrt(n * th / 3.0)[:, None]
+    # sum |v|^2 = n |u|^2 + 2 u . (sum c) + sum |c|^2, thermal part with chi-square noise
+    sc = s1 - n[:, None] * u
+    s2 = n * (u * u).sum(1) + 2 * (u * sc).sum(1) + th * (n + rng.normal(size=C_) * np.sqrt(2.0 * n / 3.0))
+    return np.column_stack([n, s1, s2])
+
+
+def test_block_moments_and_moment_fields():
+    X, Y, Z = ens.GRID
+    C_ = X * Y * Z
+    st = np.zeros((C_, 5))
+    st[:, 0] = 2.0
+    st[:, 1] = 2.0 * 300.0  # u = 300
+    st[:, 4] = 2.0 * (300.0 ** 2 + 3 * 1e4)  # 3kT/m = 3e4
+    bm = ws.block_moments(st, 4)
+    assert bm.shape == (X * Y * Z // 64, 5)
+    np.testing.assert_allclose(bm[:, 0], 128.0)
+    f = ws.moment_fields(bm, 16)
+    np.testing.assert_allclose(f[:, 0], 8.0)
+    np.testing.assert_allclose(f[:, 1], 300.0)
+    np.testing.assert_allclose(f[:, 2], 3e4, rtol=1e-12)
+    # a block where cells move differently: the spread of cell means counts as thermal
+    st2 = st.copy()
+    st2[0::2, 1] = 2.0 * 200.0
+    st2[0::2, 4] = 2.0 * (200.0 ** 2 + 3e4)
+    f2 = ws.moment_fields(ws.block_moments(st2, 4), 16)
+    np.testing.assert_allclose(f2[:, 1], 250.0)
+    np.testing.assert_allclose(f2[:, 2], 3e4 + 50.0 ** 2, rtol=1e-12)
+
+
+def _run_detector(drift, seed, halves=12):
+    X, Y, Z = ens.GRID
+    C_ = X * Y * Z
+    rng = np.random.default_rng(seed)
+    lam = np.full(C_, 8.0)
+    u0 = np.zeros((C_, 3))
+    u0[:, 0] = 1500.0
+    th = np.full(C_, 3 * 1.2e5)
+    det = ws.SteadyDetector(np.ones(C_ // 64, bool), 4, 20)
+    cum = np.zeros((C_, 5))
+    fired = []
+    for h in range(halves):
+        u = u0 * (1.0 + drift * h)
+        cum = cum + _half_window(rng, lam, u, th, 20)
+        if det.feed(cum):
+            fired.append(h)
+    return fired, det
+
+
+def test_steady_detector_fires_on_a_stationary_field():
+    for seed in range(3):
+        fired, det = _run_detector(0.0, seed)
+        # the first test is at the end of the second chunk (half windows 0..3)
+        assert fired and fired[0] == 3
+        d_c, d_h = det.last
+        # pure noise: chunk-to-chunk change ~ sqrt(1/2) of the half-to-half change
+        np.testing.assert_allclose(d_c / d_h, np.sqrt(0.5), rtol=0.15)
+
+
+def test_steady_detector_waits_while_the_field_drifts():
+    # 2% speed drift per half window is far above the noise of a chunk
+    fired, det = _run_detector(0.02, 0)
+    assert fired == []
+    d_c, d_h = det.last
+    assert d_c[1] > 1.5 * d_h[1]
+
+
+def test_steady_detector_waits_for_a_drift_in_one_field():
+    # density drifting alone holds the start back (the test pools the fields)
+    X, Y, Z = ens.GRID
+    C_ = X * Y * Z
+    rng = np.random.default_rng(5)
+    u = np.zeros((C_, 3))
+    u[:, 0] = 1500.0
+    th = np.full(C_, 3 * 1.2e5)
+    det = ws.SteadyDetector(np.ones(C_ // 64, bool), 4, 20)
+    cum = np.zeros((C_, 5))
+    fired = []
+    for h in range(10):
+        cum = cum + _half_window(rng, np.full(C_, 8.0 * (1 + 0.03 * h)), u, th, 20)
+        fired.append(det.feed(cum))
+    assert not any(fired)
+    assert det.last[0][1] ** 2 <= det.last[1][1] ** 2  # the speed alone would have passed
