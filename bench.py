@@ -47,8 +47,12 @@ WORKLOADS = {
 }
 
 
-def make_case(name: str, n_part: int | None = None):
+def make_case(name: str, n_part: int | None = None, cg_refine: int | None = None):
+    """Settings, body mesh and collision-grid dimensions of a workload. The
+    collision grid covers the body's bounding box with cells `cg_refine`
+    times finer than the flow cells (default: the tuned value, else 1)."""
     import cudadsmc_amd as dm
+    from cudadsmc_amd import ensemble as ens
 
     w = WORKLOADS[name]
     s = dm.settings_default()
@@ -70,9 +74,25 @@ def make_case(name: str, n_part: int | None = None):
         mesh = dm.icosphere((0.1, 0.1, 0.1), 0.025, 5)
     else:
         mesh = dm.icosphere((0.1, 0.1, 0.1), 0.025, 3)
+    if cg_refine is None:
+        cg_refine = int(ens.tuned(name, 1, "cg_refine"))
     lo, hi = mesh.vertices.min(0), mesh.vertices.max(0)
-    cgrid = tuple(int(max(1, np.ceil((hi[k] - lo[k]) / s.cell_size[k]))) for k in range(3))
+    cgrid = tuple(int(max(1, np.ceil(cg_refine * (hi[k] - lo[k]) / s.cell_size[k]))) for k in range(3))
     return s, mesh, cgrid
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per k_tile launch of this workload as measured by rocprofv3
+    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a plain copy of
+    known size) and stored next to the profile summaries; None if absent."""
+    p = ROOT / "profiles" / f"k_tile_{workload}_traffic.txt"
+    try:
+        for line in p.read_text().splitlines():
+            if line.startswith("bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def cpu_baseline(seconds: float = 15.0) -> dict:
@@ -173,6 +193,8 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized buffer 5 times (calibrates HBM byte counters)")
     ap.add_argument("--ensemble-steps", type=int, default=300)
+    ap.add_argument("--cg-refine", type=int, default=None,
+                    help="collision-grid cells per flow cell along each axis (default: tuned, else 1)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition is "
@@ -185,20 +207,28 @@ def main() -> int:
         n = args.copy_probe * (1 << 20) // 4
         a = torch.ones(n, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
-        for _ in range(5):
+        b.copy_(a)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(4):
             b.copy_(a)
+        e1.record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": 4 * n, "copies": 5}))
+        ms = e0.elapsed_time(e1) / 4
+        print(json.dumps({"copy_probe_bytes": 4 * n, "copies": 5, "copy_ms": ms,
+                          "copy_GBps": 2 * 4 * n / (ms * 1e-3) / 1e9}))
         return 0
 
     import cudadsmc_amd as dm
     from cudadsmc_amd import ensemble as ens
 
-    s, mesh, cgrid = make_case(args.workload)
+    s, mesh, cgrid = make_case(args.workload, cg_refine=args.cg_refine)
     s.n_streams = int(ens.tuned(args.workload, 0))  # 0: the engine default
     t_setup = time.perf_counter()
     sol = dm.Solver()
     sol.Init(s)
+    if ens.tuned(args.workload, 0, "scatter_grid"):
+        sol.set_scatter_grid(int(ens.tuned(args.workload, 0, "scatter_grid")))
     sol.SetModel(mesh, grid=cgrid)
     n0 = sol.InitSystemState(tuple(s.stream_velocity))
     sol.synchronize()
@@ -256,6 +286,7 @@ def main() -> int:
                         f"dt={s.dt:.1e}",
             "particles_avg": n_avg,
             "cells": cells,
+            "collision_grid": list(cgrid),
             "parallelism": "single GPU",
         },
         "pct_hbm_peak": 100.0 * hbm_step / HBM_PEAK,
@@ -264,7 +295,7 @@ def main() -> int:
                                "walls": float(tm[6]), "inject": float(tm[3]), "total": float(tm[4]),
                                "timed_steps": int(tm[5])},
         "roofline": {"bound": "hbm", "achieved": tile_gbs / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": tile_gbs / HBM_PEAK, "traffic": None,
+                     "frac": tile_gbs / HBM_PEAK, "traffic": measured_traffic(args.workload),
                      "kernel": "k_tile (56 B/particle + 20 B/cell algorithmic)"},
         "validate_bad_particles": int(bad),
     }

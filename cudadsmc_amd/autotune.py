@@ -11,6 +11,11 @@ Two parameters are tuned by timing the real case on the GPU:
 * ``scatter_grid``: workgroups of the grid-stride scatter kernel (does not
   change results).
 
+For the large single case it also picks the collision-grid refinement (cells
+of the triangle grid per flow cell along each axis): a finer grid holds
+fewer triangles per cell and marks fewer flow cells as near the body, at the
+price of more cells to traverse. Nearest-hit results do not depend on it.
+
 The tuner builds one solver per candidate (``make(n_streams) -> Solver``,
 loaded and ready), warms it up, times steps between device
 synchronisations, and keeps the fastest. Results can be cached in a JSON
@@ -128,8 +133,8 @@ def main(argv=None) -> None:
 
             from . import Solver
 
-            def make(ns):
-                s, mesh, cg = make_case("c3")
+            def make(ns, refine):
+                s, mesh, cg = make_case("c3", cg_refine=refine)
                 s.n_streams = ns
                 sol = Solver()
                 sol.Init(s)
@@ -137,8 +142,20 @@ def main(argv=None) -> None:
                 sol.InitSystemState(tuple(s.stream_velocity))
                 return sol
 
-            res = tune_solver(make, streams=(262144, 524288, 1048576, 2097152), scatter=(4096, 8192, 16384),
-                              warmup=20, steps=40)
+            # collision-grid refinement first (engine-default streams), then
+            # the launch layout on the chosen grid
+            rtab = []
+            for refine in (1, 2, 3, 4):
+                sol = make(0, refine)
+                dt = sol.settings.dt
+                t = _time_steps(lambda k: sol.DoBenchmark(k, dt), sol.synchronize, 20, 40)
+                sol.close()
+                rtab.append({"cg_refine": refine, "ms_per_step": 1e3 * t})
+            refine = min(rtab, key=lambda r: r["ms_per_step"])["cg_refine"]
+            res = tune_solver(lambda ns: make(ns, refine), streams=(262144, 524288, 1048576, 2097152),
+                              scatter=(1024, 2048, 4096, 8192), warmup=20, steps=40)
+            res["cg_refine"] = refine
+            res["table"] = rtab + res["table"]
         else:
             raise SystemExit(f"unknown tuning target {what}")
         db[what] = res

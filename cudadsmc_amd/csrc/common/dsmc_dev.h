@@ -59,6 +59,7 @@ enum Face : uint32_t { kFaceSpecular = 0, kFacePeriodic = 1, kFaceFreestream = 2
 struct InstDesc {
   // ---- simulation grid
   float gmin[3], gmax[3], cell[3];
+  float inv_cell[3];   // 1 / cell
   uint32_t dim[3];
   uint32_t ncells;
   uint32_t tile_cells;    // cells per tile: a power of two in [32, kTileCells]
@@ -120,6 +121,7 @@ struct InstDesc {
   // ---- scalars and per-workgroup accumulators
   uint32_t* ctr;        // [0] stored, [1] sorted, [2] injected particles, [3] status bits, [4] step number,
                         // [5] tiles queued for the wall pass
+  const uint32_t* tile_order;  // tiles in locality order [ntiles] (null: identity)
   uint32_t* near_tiles; // PHYSICS with a body: tiles with deferred particles this step [ntiles]
   double* tile_acc;     // PHYSICS with a body: wall {Fx, Fy, Fz, heat} sums per tile [ntiles][4]
   uint32_t wall_key[2]; // Philox key of the wall re-emission draws
@@ -137,6 +139,34 @@ struct InstDesc {
   uint32_t n_face_cells;
   uint32_t* face_count; // [n_face_cells + 1] counts then offsets
 };
+
+// Walk of workgroup vw (of a grid of G) over the positions of the tile
+// order. The hardware deals workgroups out to the 8 XCDs round-robin (XCD =
+// vw mod 8). With G a multiple of 8, the order is cut into patches of
+// kTilePatch positions dealt out to the XCDs round-robin as well, and the
+// G/8 workgroups of an XCD take the positions of its patches in turn: the
+// tiles of one patch (neighbours in space) run together under one L2, and
+// every XCD gets patches from all over the domain (the same share of
+// expensive near-body tiles). Otherwise workgroup vw takes positions vw,
+// vw + G, ... Together with the stream of each lane (vw * 256 + lane) the
+// walk fixes the RNG assignment.
+constexpr uint32_t kTilePatch = 128;
+
+struct TileWalk {
+  uint32_t e, step, xcd;
+  bool xcd_walk;
+};
+
+__host__ __device__ inline TileWalk tile_walk_begin(uint32_t G, uint32_t vw) {
+  if (G % 8u == 0u) return {vw / 8u, G / 8u, vw % 8u, true};
+  return {vw, G, 0u, false};
+}
+
+// Position of the walk's current element (may be >= ntiles: the walk ends).
+__host__ __device__ inline uint32_t tile_walk_pos(const TileWalk& w) {
+  if (!w.xcd_walk) return w.e;
+  return ((w.e / kTilePatch) * 8u + w.xcd) * kTilePatch + w.e % kTilePatch;
+}
 
 // Per virtual-workgroup accumulator slots (doubles).
 enum AccSlot {

@@ -32,6 +32,18 @@ __device__ __forceinline__ uint32_t cell_key(const InstDesc& I, float x, float y
   return iz * I.dim[0] * I.dim[1] + iy * I.dim[0] + ix;
 }
 
+// PHYSICS cell key: the offset times the float reciprocal of the cell size
+// (one multiply per axis instead of an IEEE divide).
+__device__ __forceinline__ uint32_t cell_key_phys(const InstDesc& I, float x, float y, float z) {
+  uint32_t ix = dm_f2u_sat((x - I.gmin[0]) * I.inv_cell[0]);
+  uint32_t iy = dm_f2u_sat((y - I.gmin[1]) * I.inv_cell[1]);
+  uint32_t iz = dm_f2u_sat((z - I.gmin[2]) * I.inv_cell[2]);
+  ix = min(ix, I.dim[0] - 1u);
+  iy = min(iy, I.dim[1] - 1u);
+  iz = min(iz, I.dim[2] - 1u);
+  return iz * I.dim[0] * I.dim[1] + iy * I.dim[0] + ix;
+}
+
 // GetCollisionCellID (dsmc_kernel.cu:154-204).
 __device__ __forceinline__ uint32_t cg_cell(const InstDesc& I, float x, float y, float z) {
   if (x < I.cg_min[0] || y < I.cg_min[1] || z < I.cg_min[2] || x > I.cg_max[0] || y > I.cg_max[1] ||
@@ -395,7 +407,7 @@ __device__ __forceinline__ uint32_t phys_faces_key(const InstDesc& I, float& px,
   if (!phys_face(px, vx, I.gmin[0], I.gmax[0], I.face_bc[0], I.face_bc[1])) return kInvalidKey;
   if (!phys_face(py, vy, I.gmin[1], I.gmax[1], I.face_bc[2], I.face_bc[3])) return kInvalidKey;
   if (!phys_face(pz, vz, I.gmin[2], I.gmax[2], I.face_bc[4], I.face_bc[5])) return kInvalidKey;
-  return cell_key(I, px, py, pz);
+  return cell_key_phys(I, px, py, pz);
 }
 
 // Free flight of a particle that is not near the body.

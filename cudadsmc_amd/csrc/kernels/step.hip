@@ -106,7 +106,8 @@ __global__ void k_keys_count(const InstDesc* __restrict__ Ip, uint32_t par, uint
   uint32_t i = base + threadIdx.x;
   uint32_t key = kInvalidKey;
   if (i < n) {
-    key = cell_key(I, I.px[par][i], I.py[par][i], I.pz[par][i]);
+    const float x = I.px[par][i], y = I.py[par][i], z = I.pz[par][i];
+    key = I.mode == kModePhysics ? cell_key_phys(I, x, y, z) : cell_key(I, x, y, z);
     I.key[par][i] = key;
   }
   count_keys(I.count, key);
@@ -325,29 +326,45 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __
 
 // =================================================================== scatter
 // perm[cursor[key[i]]++] = i with one atomic per run of equal keys in a wave.
+// Each thread takes kScatterBatch positions 256 apart per pass: their key
+// loads, then their atomics, are issued together so that the latencies of
+// the returning atomics overlap.
+constexpr uint32_t kScatterBatch = 4;
+
 __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip, uint32_t par) {
   const InstDesc& I = Ip[blockIdx.y];
   const uint32_t n = I.ctr[0];
   const uint32_t* keys = I.key[par];
   const uint32_t lane = lane_id();
-  for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
-    uint32_t i = base + threadIdx.x;
-    uint32_t key = i < n ? keys[i] : kInvalidKey;
-    uint32_t prev = __shfl_up(key, 1);
-    bool head = (lane == 0) || (key != prev);
-    uint64_t heads = __ballot(head);
-    uint32_t slot = 0;
-    if (head && key != kInvalidKey) {
-      uint64_t above = heads & ~((2ull << lane) - 1ull);
-      if (lane == 63) above = 0;
-      uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
-      slot = atomicAdd(&I.cursor[key], next - lane);
+  const uint64_t below_incl = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  for (uint32_t base = blockIdx.x * (256u * kScatterBatch); base < n; base += gridDim.x * (256u * kScatterBatch)) {
+    uint32_t key[kScatterBatch], slot[kScatterBatch];
+    uint64_t heads[kScatterBatch];
+#pragma unroll
+    for (uint32_t k = 0; k < kScatterBatch; ++k) {
+      const uint32_t i = base + k * 256u + threadIdx.x;
+      key[k] = i < n ? keys[i] : kInvalidKey;
     }
-    uint64_t mine = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
-    uint32_t hl = 63u - (uint32_t)__clzll((long long)mine);
-    uint32_t hslot = __shfl(slot, (int)hl);
-    uint32_t dst = hslot + (lane - hl);
-    if (key != kInvalidKey && dst < I.cap) I.perm[dst] = i;
+#pragma unroll
+    for (uint32_t k = 0; k < kScatterBatch; ++k) {
+      const uint32_t prev = __shfl_up(key[k], 1);
+      const bool head = (lane == 0) || (key[k] != prev);
+      heads[k] = __ballot(head);
+      slot[k] = 0;
+      if (head && key[k] != kInvalidKey) {
+        uint64_t above = heads[k] & ~below_incl;
+        uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
+        slot[k] = atomicAdd(&I.cursor[key[k]], next - lane);
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kScatterBatch; ++k) {
+      const uint32_t i = base + k * 256u + threadIdx.x;
+      uint32_t hl = 63u - (uint32_t)__clzll((long long)(heads[k] & below_incl));
+      uint32_t hslot = __shfl(slot[k], (int)hl);
+      uint32_t dst = hslot + (lane - hl);
+      if (key[k] != kInvalidKey && dst < I.cap) I.perm[dst] = i;
+    }
   }
 }
 
@@ -620,7 +637,10 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
   CollAcc cacc = {0.0, 0.0, 0.0};
   if (vw == 0 && tid == 0) I.ctr[0] = I.ctr[1];
 
-  for (uint32_t tile = vw; tile < I.ntiles; tile += I.n_vwg) {
+  for (TileWalk w = tile_walk_begin(I.n_vwg, vw);; w.e += w.step) {
+    const uint32_t j = tile_walk_pos(w);
+    if (j >= I.ntiles) break;
+    const uint32_t tile = I.tile_order ? I.tile_order[j] : j;
     const uint32_t c0 = tile * I.tile_cells;
     const uint32_t nc = min(I.tile_cells, I.ncells - c0);
     s_start[tid] = I.cell_start[c0 + min(tid, nc)];
@@ -998,7 +1018,7 @@ __global__ void k_inject_generate(const InstDesc* __restrict__ Ip, uint32_t opar
       if (idx >= I.cap) continue;
       I.px[opar][idx] = x; I.py[opar][idx] = y; I.pz[opar][idx] = zz;
       I.vx[opar][idx] = v[0]; I.vy[opar][idx] = v[1]; I.vz[opar][idx] = v[2];
-      uint32_t key = cell_key(I, x, y, zz);
+      uint32_t key = cell_key_phys(I, x, y, zz);
       I.key[opar][idx] = key;
       atomicAdd(&I.count[key], 1u);
     }

@@ -222,6 +222,7 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
     I.gmin[a] = s.bmin[a];
     I.gmax[a] = s.bmax[a];
     I.cell[a] = s.cell_size[a];
+    I.inv_cell[a] = 1.0f / s.cell_size[a];
     I.dim[a] = (uint32_t)s.grid_dim[a];
   }
   I.ncells = C;
@@ -413,7 +414,8 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
     HIPC(hipMemcpyAsync(I.taus, ts.data(), ts.size() * 4, hipMemcpyHostToDevice, st));
   }
   HIPC(hipStreamSynchronize(st));
-  h->scatter_grid = std::min<uint32_t>((cap + 255) / 256, 8192u);
+  // k_scatter covers 1024 positions per workgroup and pass
+  h->scatter_grid = std::min<uint32_t>((cap + 1023) / 1024, 4096u);
   if (phys) {
     // F_n and the inflow per face for the whole box; dsmc_set_geometry
     // redoes both for the fluid volume
@@ -546,9 +548,42 @@ int dsmc_set_geometry(dsmc_solver* h, const float* verts, uint32_t nv, const uin
   return DSMC_OK;
 }
 
+// Tiles in locality order: sorted by (z / B, y / B, z mod B, y mod B, x) of
+// their first cell, so that consecutive positions of the order cover B x B
+// patches of cell rows. B is the largest power of two for which a patch
+// holds at most 128 tiles, about what one XCD keeps resident (32 CUs x 4
+// workgroups): the tiles a particle can come from in one step then mostly
+// run together on that XCD.
+static std::vector<uint32_t> tile_locality_order(uint32_t X, uint32_t Y, uint32_t Z, uint32_t tc, uint32_t ntiles) {
+  (void)Z;
+  uint32_t B = 1;
+  while ((uint64_t)(2 * B) * (2 * B) * X <= 128ull * tc && 2 * B <= std::max(Y, Z)) B *= 2;
+  std::vector<std::pair<uint64_t, uint32_t>> k(ntiles);
+  for (uint32_t t = 0; t < ntiles; ++t) {
+    const uint64_t c = (uint64_t)t * tc;
+    const uint64_t x = c % X, y = (c / X) % Y, z = c / ((uint64_t)X * Y);
+    const uint64_t key = ((((z / B) * ((Y + B - 1) / B) + y / B) * B + z % B) * B + y % B) * X + x;
+    k[t] = {key, t};
+  }
+  std::sort(k.begin(), k.end());
+  std::vector<uint32_t> order(ntiles);
+  for (uint32_t t = 0; t < ntiles; ++t) order[t] = k[t].second;
+  return order;
+}
+
 static int finish_loading(dsmc_solver* h, uint32_t n, uint32_t* n_out) {
   InstDesc& I = h->h;
   hipStream_t st = h->stream;
+  if (!I.tile_order) {
+    // the tile count is fixed from here on
+    std::vector<uint32_t> order = tile_locality_order(I.dim[0], I.dim[1], I.dim[2], I.tile_cells, I.ntiles);
+    uint32_t* d = nullptr;
+    CHECK_RC(h->alloc(&d, I.ntiles));
+    HIPC(hipMemcpyAsync(d, order.data(), (size_t)I.ntiles * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipStreamSynchronize(st));
+    I.tile_order = d;
+    CHECK_RC(h->upload_desc());
+  }
   if (h->physics() && I.has_geom && !I.near_tiles) {
     // wall pass queue and per-tile wall force sums (tile count is fixed now)
     CHECK_RC(h->alloc(&I.near_tiles, I.ntiles));
@@ -1245,6 +1280,7 @@ int dsmc_op_cell_ids(const dsmc_settings* s, const float* x, const float* y, con
     I.gmin[a] = s->bmin[a];
     I.gmax[a] = s->bmax[a];
     I.cell[a] = (s->bmax[a] - s->bmin[a]) / (float)s->grid_dim[a];
+    I.inv_cell[a] = 1.0f / I.cell[a];
     I.dim[a] = (uint32_t)s->grid_dim[a];
   }
   dsmc_launch::cell_ids(I, x, y, z, n, keys, (hipStream_t)stream);

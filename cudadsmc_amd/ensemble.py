@@ -33,11 +33,12 @@ ICO_SUBDIV = 3  # 1280 triangles
 TUNED = Path(__file__).resolve().parent / "tuned.json"
 
 
-def tuned(key: str, default):
+def tuned(key: str, default, field: str = "n_streams"):
     try:
-        return json.loads(TUNED.read_text()).get(key, {}).get("n_streams", default)
-    except (OSError, ValueError):
+        v = json.loads(TUNED.read_text()).get(key, {}).get(field)
+    except (OSError, ValueError, AttributeError):
         return default
+    return default if v is None else v
 
 
 @dataclass

@@ -55,6 +55,8 @@ def lib() -> C.CDLL:
         "oracle_get_rng": ([P, pU, U32], I32),
         "oracle_set_ntc": ([P, pF, U32], I32),
         "oracle_get_ntc": ([P, pF, U32], I32),
+        "oracle_tile_order": ([P, pU, U32], I32),
+        "oracle_tile_walk": ([P, U32, pU, U32], I32),
         "oracle_set_state": ([P, pF, pF, U32], I32),
         "oracle_n_stored": ([P], U32),
         "oracle_get_state": ([P, pF, pF, pU, U32], I32),
@@ -257,6 +259,18 @@ class OracleSim:
     def set_ntc(self, ntc: np.ndarray) -> None:
         a = np.ascontiguousarray(ntc, np.float32).ravel()
         assert lib().oracle_set_ntc(self._o, fp(a), self.C) == 0
+
+    def tile_order(self) -> np.ndarray:
+        a = np.zeros(self.C, np.uint32)
+        k = lib().oracle_tile_order(self._o, up(a), self.C)
+        assert k >= 0
+        return a[:k].copy()
+
+    def tile_walk(self, vw: int) -> np.ndarray:
+        a = np.zeros(self.C, np.uint32)
+        k = lib().oracle_tile_walk(self._o, vw, up(a), self.C)
+        assert k >= 0
+        return a[:k].copy()
 
     def ntc(self) -> np.ndarray:
         a = np.zeros(2 * self.C, np.float32)

@@ -19,11 +19,19 @@
 // and the PHYSICS extensions (VHS NTC, nearest-hit walls by brute force over
 // all triangles, diffuse re-emission, freestream faces and injection).
 //
-// The engine's RNG stream assignment is part of the algorithm: cell c of
-// tile t draws from stream ((t mod G) * threads + c mod tile_cells) and the
-// move of tile position q from stream ((t mod G) * threads + q mod threads),
-// G = n_streams / threads; loading and injection draw from stream (item mod
-// n_streams). The oracle follows that assignment with plain loops.
+// The engine's RNG stream assignment is part of the algorithm. Tiles are
+// visited in a locality order (sorted by (z / B, y / B, z mod B, y mod B, x)
+// of their first cell, B the largest power of two with B^2 x <= 128
+// tile_cells). When G = n_streams / threads is a multiple of 8, the order is
+// cut into patches of 128 positions, patch p belongs to "XCD" p mod 8, and
+// workgroup w takes elements w / 8, w / 8 + G / 8, ... of the concatenated
+// patches of XCD w mod 8 (element e is position
+// ((e / 128) * 8 + w mod 8) * 128 + e mod 128) until a position passes the
+// tile count; else workgroup w takes positions w, w + G, ...
+// Cell lane l of a tile draws from stream w * threads + l and
+// the move of tile position q from stream w * threads + q mod threads;
+// loading and injection draw from stream (item mod n_streams). The oracle
+// follows that assignment with plain loops.
 #include "dsmc_cpu.h"
 
 #include <algorithm>
@@ -195,6 +203,17 @@ struct oracle_sim {
     uint32_t iy = dm_f2u_sat((py - gmin[1]) / cell[1]);
     uint32_t iz = dm_f2u_sat((pz - gmin[2]) / cell[2]);
     return std::min(iz, Z - 1) * X * Y + std::min(iy, Y - 1) * X + std::min(ix, X - 1);
+  }
+  // PHYSICS: offset times the float reciprocal of the cell size
+  uint32_t cell_key_phys(float px, float py, float pz) const {
+    const float r0 = 1.0f / cell[0], r1 = 1.0f / cell[1], r2 = 1.0f / cell[2];
+    uint32_t ix = dm_f2u_sat((px - gmin[0]) * r0);
+    uint32_t iy = dm_f2u_sat((py - gmin[1]) * r1);
+    uint32_t iz = dm_f2u_sat((pz - gmin[2]) * r2);
+    return std::min(iz, Z - 1) * X * Y + std::min(iy, Y - 1) * X + std::min(ix, X - 1);
+  }
+  uint32_t key_of(float px, float py, float pz) const {
+    return phys ? cell_key_phys(px, py, pz) : cell_key(px, py, pz);
   }
   uint32_t cg_cell(float px, float py, float pz) const {  // GetCollisionCellID
     if (px < cgmin[0] || py < cgmin[1] || pz < cgmin[2] || px > cgmax[0] || py > cgmax[1] || pz > cgmax[2])
@@ -396,7 +415,7 @@ struct oracle_sim {
     if (!face(px, ux, gmin[0], gmax[0], s.face_bc[0], s.face_bc[1])) return INV;
     if (!face(py, uy, gmin[1], gmax[1], s.face_bc[2], s.face_bc[3])) return INV;
     if (!face(pz, uz, gmin[2], gmax[2], s.face_bc[4], s.face_bc[5])) return INV;
-    return cell_key(px, py, pz);
+    return cell_key_phys(px, py, pz);
   }
 
   uint32_t move_free(float& px, float& py, float& pz, float& ux, float& uy, float& uz) const {
@@ -575,6 +594,43 @@ struct oracle_sim {
     return false;
   }
 
+  // ------------------------------------------------------ tile schedule
+  // Locality order of the tiles (see the header).
+  std::vector<uint32_t> tile_order(uint32_t ntiles) const {
+    uint32_t B = 1;
+    while (uint64_t(4) * B * B * X <= uint64_t(128) * TC && 2 * B <= std::max(Y, Z)) B *= 2;
+    const uint64_t yp = (Y + B - 1) / B;
+    std::vector<std::pair<uint64_t, uint32_t>> keyed;
+    keyed.reserve(ntiles);
+    for (uint32_t t = 0; t < ntiles; ++t) {
+      const uint64_t c = uint64_t(t) * TC;
+      const uint64_t cx = c % X, cy = (c / X) % Y, cz = c / (uint64_t(X) * Y);
+      const uint64_t patch = (cz / B) * yp + cy / B;
+      keyed.push_back({((patch * B + cz % B) * B + cy % B) * X + cx, t});
+    }
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<uint32_t> out;
+    out.reserve(ntiles);
+    for (const auto& e : keyed) out.push_back(e.second);
+    return out;
+  }
+
+  // Positions of the order that workgroup vw visits, in visiting order.
+  std::vector<uint32_t> tile_positions(uint32_t vw, uint32_t ntiles) const {
+    std::vector<uint32_t> js;
+    if (G % 8 == 0) {
+      const uint64_t P = 128, x = vw % 8;
+      for (uint64_t e = vw / 8;; e += G / 8) {
+        const uint64_t j = ((e / P) * 8 + x) * P + e % P;
+        if (j >= ntiles) break;
+        js.push_back((uint32_t)j);
+      }
+    } else {
+      for (uint64_t j = vw; j < ntiles; j += G) js.push_back((uint32_t)j);
+    }
+    return js;
+  }
+
   // ------------------------------------------------------------- step
   void step(bool sample) {
     ++stepno;  // the engine's step counter (counter of the wall draws)
@@ -601,10 +657,12 @@ struct oracle_sim {
     std::vector<uint32_t> nkey(N);
     hits.assign(N, 0);
     const uint32_t ntiles = (C + TC - 1) / TC;
+    const std::vector<uint32_t> order = tile_order(ntiles);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t vw = 0; vw < (int64_t)G; ++vw) {
       double* a = &acc[(size_t)vw * 7];
-      for (uint32_t tile = (uint32_t)vw; tile < ntiles; tile += G) {
+      for (const uint32_t j : tile_positions((uint32_t)vw, ntiles)) {
+        const uint32_t tile = order[j];
         uint32_t c0 = tile * TC, nc = std::min(TC, C - c0);
         uint32_t p0 = cell_start[c0], p1 = cell_start[c0 + nc];
         // NTC, one stream per cell lane
@@ -759,7 +817,7 @@ struct oracle_sim {
         uint32_t idx = base + start[g] + j;
         x[idx] = px; y[idx] = py; z[idx] = pz;
         vx[idx] = v[0]; vy[idx] = v[1]; vz[idx] = v[2];
-        key[idx] = cell_key(px, py, pz);
+        key[idx] = cell_key_phys(px, py, pz);
       }
     }
     n_store = base + tot;
@@ -840,7 +898,7 @@ struct oracle_sim {
       n_store = n;
     }
     key.resize(n_store);
-    for (uint32_t i = 0; i < n_store; ++i) key[i] = cell_key(x[i], y[i], z[i]);
+    for (uint32_t i = 0; i < n_store; ++i) key[i] = key_of(x[i], y[i], z[i]);
   }
 };
 
@@ -852,6 +910,24 @@ void oracle_mt_generic(uint32_t n, uint32_t m, uint32_t a, uint32_t lower_mask, 
   g.n = n; g.m = m; g.a = a; g.lower = lower_mask; g.u = u; g.s = s; g.b = b; g.t = t; g.c = c; g.l = l;
   g.seed(seed);
   for (uint32_t k = 0; k < n_out; ++k) out[k] = g.next();
+}
+
+int oracle_tile_order(oracle_sim* o, uint32_t* out, uint32_t n) {
+  if (!o || !out) return -1;
+  const uint32_t ntiles = (o->C + o->TC - 1) / o->TC;
+  if (n < ntiles) return -1;
+  std::vector<uint32_t> ord = o->tile_order(ntiles);
+  std::copy(ord.begin(), ord.end(), out);
+  return (int)ntiles;
+}
+
+int oracle_tile_walk(oracle_sim* o, uint32_t vw, uint32_t* out, uint32_t n) {
+  if (!o || !out || vw >= o->G) return -1;
+  const uint32_t ntiles = (o->C + o->TC - 1) / o->TC;
+  std::vector<uint32_t> js = o->tile_positions(vw, ntiles);
+  if (js.size() > n) return -1;
+  std::copy(js.begin(), js.end(), out);
+  return (int)js.size();
 }
 
 void oracle_mt_streams(const uint32_t* a, const uint32_t* b, const uint32_t* c, const uint32_t* seeds,
@@ -1115,7 +1191,7 @@ int oracle_set_state(oracle_sim* o, const float* pos, const float* vel, uint32_t
   o->vy.assign(vel + n, vel + 2 * (size_t)n);
   o->vz.assign(vel + 2 * (size_t)n, vel + 3 * (size_t)n);
   o->key.resize(n);
-  for (uint32_t i = 0; i < n; ++i) o->key[i] = o->cell_key(o->x[i], o->y[i], o->z[i]);
+  for (uint32_t i = 0; i < n; ++i) o->key[i] = o->key_of(o->x[i], o->y[i], o->z[i]);
   o->n_store = n;
   return 0;
 }

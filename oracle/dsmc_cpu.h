@@ -62,6 +62,12 @@ int oracle_set_rng(oracle_sim* o, const uint32_t* words, uint32_t n_words);
 int oracle_get_rng(oracle_sim* o, uint32_t* words, uint32_t n_words);
 int oracle_set_ntc(oracle_sim* o, const float* ntc, uint32_t n_cells);
 int oracle_get_ntc(oracle_sim* o, float* ntc, uint32_t n_cells);
+/* The locality order in which the step visits the tiles; returns the tile
+ * count, or -1 when n is too small. */
+int oracle_tile_order(oracle_sim* o, uint32_t* out, uint32_t n);
+/* Positions of that order visited by workgroup vw, in order; returns their
+ * number, or -1. */
+int oracle_tile_walk(oracle_sim* o, uint32_t vw, uint32_t* out, uint32_t n);
 /* State in storage order: pos/vel as 3 blocks of n. */
 int oracle_set_state(oracle_sim* o, const float* pos, const float* vel, uint32_t n);
 uint32_t oracle_n_stored(oracle_sim* o);
