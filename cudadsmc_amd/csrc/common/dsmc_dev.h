@@ -140,6 +140,49 @@ struct InstDesc {
   uint32_t* face_count; // [n_face_cells + 1] counts then offsets
 };
 
+// Global-memory view of a pointer held in a descriptor. Pointers loaded from
+// memory are generic to the compiler, so accesses through them compile to
+// FLAT instructions, which count against the LDS wait counter as well and
+// serialise LDS work behind outstanding global loads; through these views
+// they compile to global_* instructions.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <class T>
+__device__ __forceinline__ gptr<T> gmem(T* p) {
+  return (gptr<T>)p;
+}
+
+// HIP's vector structs cannot be copied out of a global-space reference: go
+// through the compiler's native vector types.
+typedef float dsmc_f2 __attribute__((ext_vector_type(2)));
+typedef float dsmc_f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t dsmc_u4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 gload(gptr<const float4> p) {
+  const dsmc_f4 v = *(gptr<const dsmc_f4>)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 gload(gptr<const float2> p) {
+  const dsmc_f2 v = *(gptr<const dsmc_f2>)p;
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ uint4 gload(gptr<const uint4> p) {
+  const dsmc_u4 v = *(gptr<const dsmc_u4>)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gstore(gptr<float4> p, float4 a) {
+  *(gptr<dsmc_f4>)p = dsmc_f4{a.x, a.y, a.z, a.w};
+}
+__device__ __forceinline__ void gstore(gptr<float2> p, float2 a) { *(gptr<dsmc_f2>)p = dsmc_f2{a.x, a.y}; }
+__device__ __forceinline__ void gstore(gptr<uint4> p, uint4 a) {
+  *(gptr<dsmc_u4>)p = dsmc_u4{a.x, a.y, a.z, a.w};
+}
+
+__device__ __forceinline__ uint32_t gatomic_add(gptr<uint32_t> p, uint32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Walk of workgroup vw (of a grid of G) over the positions of the tile
 // order. The hardware deals workgroups out to the 8 XCDs round-robin (XCD =
 // vw mod 8). With G a multiple of 8, the order is cut into patches of

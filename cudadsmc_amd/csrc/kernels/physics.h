@@ -293,11 +293,11 @@ __device__ __forceinline__ bool phys_nearest_hit(const InstDesc& I, float ox, fl
   const int max_iter = (int)(I.cg_dim[0] + I.cg_dim[1] + I.cg_dim[2]) + 3;
   for (int it = 0; it < max_iter; ++it) {
     uint32_t cell = (uint32_t)az.c * I.cg_dim[0] * I.cg_dim[1] + (uint32_t)ay.c * I.cg_dim[0] + (uint32_t)ax.c;
-    if (I.cg_flag[cell] == kCgSurface) {
-      uint32_t s = I.cg_start[cell], e = I.cg_end[cell];
+    if (gmem(I.cg_flag)[cell] == kCgSurface) {
+      uint32_t s = gmem(I.cg_start)[cell], e = gmem(I.cg_end)[cell];
       // per-entry records {v0 + triangle id, e1, e2, n}: one contiguous
       // 64-byte read per candidate, no dependent index loads
-      const float4* rec = I.cg_rec + 4u * s;
+      gptr<const float4> rec = gmem<const float4>(I.cg_rec) + 4u * s;
       // kRecBatch records in flight before any is tested
       for (uint32_t i = s; i < e; i += kRecBatch, rec += 4 * kRecBatch) {
         float4 ra[kRecBatch][4];
@@ -305,7 +305,7 @@ __device__ __forceinline__ bool phys_nearest_hit(const InstDesc& I, float ox, fl
         for (uint32_t k = 0; k < kRecBatch; ++k) {
           if (i + k < e) {
 #pragma unroll
-            for (int w = 0; w < 4; ++w) ra[k][w] = rec[4 * k + w];
+            for (int w = 0; w < 4; ++w) ra[k][w] = gload(rec + 4 * k + w);
           }
         }
 #pragma unroll
@@ -438,7 +438,7 @@ __device__ __forceinline__ uint32_t move_walls(const InstDesc& I, float& px, flo
     float s;
     uint32_t tri;
     if (!phys_nearest_hit(I, px, py, pz, dx, dy, dz, dist, skip, s, tri)) break;
-    float4 n = I.tn[tri];
+    float4 n = gload(gmem<const float4>(I.tn) + tri);
     float hx = px + dx * s, hy = py + dy * s, hz = pz + dz * s;
     float ux, uy, uz;
     if (I.vm_wall > 0.0f) {

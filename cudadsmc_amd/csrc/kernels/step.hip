@@ -36,21 +36,22 @@ constexpr uint32_t kScanTile = kScanThreads * kScanItems;  // 4096
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 __device__ __forceinline__ void mt_load(const InstDesc& I, uint32_t s, Mt& m) {
+  const gptr<uint32_t> st = gmem(I.mt_state);
 #pragma unroll
-  for (int w = 0; w < kMtN; ++w) m.st[w] = I.mt_state[(uint32_t)w * I.n_streams + s];
-  m.a = I.mt_a[s];
-  m.b = I.mt_b[s];
-  m.c = I.mt_c[s];
+  for (int w = 0; w < kMtN; ++w) m.st[w] = st[(uint32_t)w * I.n_streams + s];
+  m.a = gmem(I.mt_a)[s];
+  m.b = gmem(I.mt_b)[s];
+  m.c = gmem(I.mt_c)[s];
 }
 
 __device__ __forceinline__ void mt_store(const InstDesc& I, uint32_t s, const Mt& m) {
 #pragma unroll
-  for (int w = 0; w < kMtN; ++w) I.mt_state[(uint32_t)w * I.n_streams + s] = m.st[w];
+  for (int w = 0; w < kMtN; ++w) gmem(I.mt_state)[(uint32_t)w * I.n_streams + s] = m.st[w];
 }
 
 // Wave-aggregated atomicAdd(&cnt[key], 1) for keys of consecutive lanes:
 // one atomic per run of equal keys. Invalid keys are skipped.
-__device__ __forceinline__ void count_keys(uint32_t* cnt, uint32_t key) {
+__device__ __forceinline__ void count_keys(gptr<uint32_t> cnt, uint32_t key) {
   uint32_t lane = lane_id();
   uint32_t prev = __shfl_up(key, 1);
   bool head = (lane == 0) || (key != prev);
@@ -59,7 +60,7 @@ __device__ __forceinline__ void count_keys(uint32_t* cnt, uint32_t key) {
     uint64_t above = heads & ~((2ull << lane) - 1ull);
     if (lane == 63) above = 0;
     uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
-    atomicAdd(&cnt[key], next - lane);
+    gatomic_add(cnt + key, next - lane);
   }
 }
 
@@ -110,14 +111,14 @@ __global__ void k_keys_count(const InstDesc* __restrict__ Ip, uint32_t par, uint
     key = I.mode == kModePhysics ? cell_key_phys(I, x, y, z) : cell_key(I, x, y, z);
     I.key[par][i] = key;
   }
-  count_keys(I.count, key);
+  count_keys(gmem(I.count), key);
 }
 
 // Plain counting of arbitrary keys (sort operator).
 __global__ void k_count_keys(const uint32_t* keys, uint32_t n, uint32_t* cnt) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t key = i < n ? keys[i] : kInvalidKey;
-  count_keys(cnt, key);
+  count_keys(gmem(cnt), key);
 }
 
 // ====================================================================== scan
@@ -334,7 +335,9 @@ constexpr uint32_t kScatterBatch = 4;
 __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip, uint32_t par) {
   const InstDesc& I = Ip[blockIdx.y];
   const uint32_t n = I.ctr[0];
-  const uint32_t* keys = I.key[par];
+  const gptr<const uint32_t> keys = gmem<const uint32_t>(I.key[par]);
+  const gptr<uint32_t> cursor = gmem(I.cursor);
+  const gptr<uint32_t> perm = gmem(I.perm);
   const uint32_t lane = lane_id();
   const uint64_t below_incl = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
   for (uint32_t base = blockIdx.x * (256u * kScatterBatch); base < n; base += gridDim.x * (256u * kScatterBatch)) {
@@ -354,7 +357,7 @@ __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip
       if (head && key[k] != kInvalidKey) {
         uint64_t above = heads[k] & ~below_incl;
         uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
-        slot[k] = atomicAdd(&I.cursor[key[k]], next - lane);
+        slot[k] = gatomic_add(cursor + key[k], next - lane);
       }
     }
 #pragma unroll
@@ -363,7 +366,7 @@ __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip
       uint32_t hl = 63u - (uint32_t)__clzll((long long)(heads[k] & below_incl));
       uint32_t hslot = __shfl(slot[k], (int)hl);
       uint32_t dst = hslot + (lane - hl);
-      if (key[k] != kInvalidKey && dst < I.cap) I.perm[dst] = i;
+      if (key[k] != kInvalidKey && dst < I.cap) perm[dst] = i;
     }
   }
 }
@@ -411,38 +414,51 @@ using dsmc_launch::kFlagKeepPerm;
 using dsmc_launch::kFlagHits;
 
 constexpr uint32_t kMoveBatch = 4;  // rounds of 256 positions whose loads are issued together
+// PHYSICS: rounds of 256 permutation entries of the next tile loaded into
+// registers during this tile's move (the next tile's first phase then only
+// stores them to LDS)
+constexpr uint32_t kPermPrefetch = 4;
 
 template <uint32_t MODE, bool LDS>
 __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint32_t flags, uint32_t tile,
                                           uint32_t c0, uint32_t nc, uint32_t p0, uint32_t np,
-                                          const uint32_t* s_start, const uint8_t* s_nearc, uint32_t* s_raw,
+                                          const uint32_t* s_start, const uint8_t* s_nearc, uint32_t tile_n,
+                                          uint32_t* sn_start, uint8_t* sn_nearc, uint32_t (&pf)[kPermPrefetch],
+                                          uint32_t* s_raw,
                                           uint32_t* s_perm, uint8_t* s_owner, float* s_vx, float* s_vy,
                                           float* s_vz, Mt& rng, CollAcc& cacc) {
   const uint32_t tid = threadIdx.x;
   const uint32_t opar = par ^ 1u;
-  const float* __restrict__ ipx = I.px[par];
-  const float* __restrict__ ipy = I.py[par];
-  const float* __restrict__ ipz = I.pz[par];
-  const float* __restrict__ ivx = I.vx[par];
-  const float* __restrict__ ivy = I.vy[par];
-  const float* __restrict__ ivz = I.vz[par];
-  float* opx = I.px[opar] + p0;
-  float* opy = I.py[opar] + p0;
-  float* opz = I.pz[opar] + p0;
-  float* ovx = I.vx[opar] + p0;
-  float* ovy = I.vy[opar] + p0;
-  float* ovz = I.vz[opar] + p0;
-  uint32_t* okey = I.key[opar] + p0;
+  const gptr<const float> ipx = gmem<const float>(I.px[par]);
+  const gptr<const float> ipy = gmem<const float>(I.py[par]);
+  const gptr<const float> ipz = gmem<const float>(I.pz[par]);
+  const gptr<const float> ivx = gmem<const float>(I.vx[par]);
+  const gptr<const float> ivy = gmem<const float>(I.vy[par]);
+  const gptr<const float> ivz = gmem<const float>(I.vz[par]);
+  const gptr<float> opx = gmem(I.px[opar] + p0);
+  const gptr<float> opy = gmem(I.py[opar] + p0);
+  const gptr<float> opz = gmem(I.pz[opar] + p0);
+  const gptr<float> ovx = gmem(I.vx[opar] + p0);
+  const gptr<float> ovy = gmem(I.vy[opar] + p0);
+  const gptr<float> ovz = gmem(I.vz[opar] + p0);
+  const gptr<uint32_t> okey = gmem(I.key[opar] + p0);
   uint32_t* raw = LDS ? s_raw : (I.perm + p0);
   uint32_t* srt = LDS ? s_perm : (I.perm_tmp + p0);
   uint8_t* own = LDS ? s_owner : (I.owner_tmp + p0);
-  float* vx = LDS ? s_vx : ovx;
-  float* vy = LDS ? s_vy : ovy;
-  float* vz = LDS ? s_vz : ovz;
+  float* vx = LDS ? s_vx : I.vx[opar] + p0;
+  float* vy = LDS ? s_vy : I.vy[opar] + p0;
+  float* vz = LDS ? s_vz : I.vz[opar] + p0;
 
   // (1) raw permutation of the tile and the owning cell of each position
   if (LDS) {
-    for (uint32_t q = tid; q < np; q += kTileThreads) raw[q] = I.perm[p0 + q];
+    if (MODE == kModePhysics) {
+#pragma unroll
+      for (uint32_t k = 0; k < kPermPrefetch; ++k)
+        if (tid + k * kTileThreads < np) raw[tid + k * kTileThreads] = pf[k];
+      for (uint32_t q = tid + kPermPrefetch * kTileThreads; q < np; q += kTileThreads) raw[q] = gmem(I.perm)[p0 + q];
+    } else {
+      for (uint32_t q = tid; q < np; q += kTileThreads) raw[q] = gmem(I.perm)[p0 + q];
+    }
   }
   if (tid < nc) {
     uint32_t a = s_start[tid] - p0, b = s_start[tid + 1] - p0;
@@ -459,6 +475,16 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
     srt[a + r] = v;
   }
   __syncthreads();
+  // the collision lane's cell state, loaded now so that its latency overlaps
+  // the velocity gathers
+  float2 nt_pf = make_float2(0.0f, 0.0f);
+  float frac_pf = 1.0f;
+  float4 st_pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (tid < nc && s_start[tid + 1] - s_start[tid] >= 2u) {
+    nt_pf = gload(gmem<const float2>(I.ntc) + c0 + tid);
+    if (MODE == kModeReference) st_pf = gload(gmem<const float4>(I.cstat) + c0 + tid);
+    else if (I.fluid_frac) frac_pf = gmem(I.fluid_frac)[c0 + tid];
+  }
   // (3) gather velocities in sorted order; the loads of kMoveBatch positions
   // are issued before any store so that their latencies overlap
   for (uint32_t q0 = tid; q0 < np; q0 += kMoveBatch * kTileThreads) {
@@ -485,32 +511,58 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
     }
   }
   __syncthreads();
+  // the next tile's cell offsets and near flags, stored to the other LDS
+  // buffer at the end of this phase: the next tile starts without a load
+  uint32_t nx_start = 0u, nx_end = 0u, ncn = 0u;
+  uint8_t nx_near = 0u;
+  if (tile_n != kInvalidKey) {
+    const uint32_t c0n = tile_n * I.tile_cells;
+    ncn = min(I.tile_cells, I.ncells - c0n);
+    nx_start = gmem(I.cell_start)[c0n + min(tid, ncn)];
+    if (tid == 0) nx_end = gmem(I.cell_start)[c0n + ncn];
+    if (MODE == kModePhysics) nx_near = (I.near_cell && tid < ncn) ? gmem(I.near_cell)[c0n + tid] : (uint8_t)0;
+  }
   // (4) NTC collisions, one lane per cell
   if (tid < nc) {
     uint32_t c = c0 + tid;
     uint32_t a = s_start[tid] - p0, b = s_start[tid + 1] - p0;
     uint32_t n = b - a;
     if (n >= 2u) {
-      float2 nt = I.ntc[c];
+      float2 nt = nt_pf;
       if (MODE == kModeReference) {
-        float4 st = I.cstat[c];
+        float4 st = st_pf;
         if (I.taus) {
-          uint4 q = reinterpret_cast<const uint4*>(I.taus)[c];
+          uint4 q = gload(gmem<const uint4>(reinterpret_cast<const uint4*>(I.taus)) + c);
           Taus t = {q.x, q.y, q.z, q.w};
           ntc_reference(I, vx + a, vy + a, vz + a, n, nt, st, t);
-          reinterpret_cast<uint4*>(I.taus)[c] = make_uint4(t.x, t.y, t.z, t.w);
+          gstore(gmem(reinterpret_cast<uint4*>(I.taus)) + c, make_uint4(t.x, t.y, t.z, t.w));
         } else {
           ntc_reference(I, vx + a, vy + a, vz + a, n, nt, st, rng);
         }
-        I.cstat[c] = st;
+        gstore(gmem(I.cstat) + c, st);
       } else {
-        float vol = I.cell_volume * (I.fluid_frac ? fmaxf(I.fluid_frac[c], kMinFluidFrac) : 1.0f);
+        float vol = I.cell_volume * (I.fluid_frac ? fmaxf(frac_pf, kMinFluidFrac) : 1.0f);
         ntc_physics(I, vx + a, vy + a, vz + a, n, vol, nt, rng, cacc);
       }
-      I.ntc[c] = nt;
+      gstore(gmem(I.ntc) + c, nt);
     }
   }
+  if (tile_n != kInvalidKey) {
+    sn_start[tid] = nx_start;
+    if (tid == 0) sn_start[ncn] = nx_end;
+    if (MODE == kModePhysics) sn_nearc[tid] = nx_near;
+  }
   __syncthreads();
+  // the next tile's first permutation entries (its offsets are in LDS now)
+  if (MODE == kModePhysics && tile_n != kInvalidKey) {
+    const uint32_t p0n = sn_start[0];
+    const uint32_t npn = sn_start[ncn] - p0n;
+#pragma unroll
+    for (uint32_t k = 0; k < kPermPrefetch; ++k) {
+      const uint32_t q = tid + k * kTileThreads;
+      pf[k] = (npn <= kLdsCap && q < npn) ? gmem(I.perm)[p0n + q] : 0u;
+    }
+  }
   // (5) move, walls, box faces, next key; rounds keep waves converged for
   // the aggregated counting
   const uint32_t rounds = (np + kTileThreads - 1u) / kTileThreads;
@@ -563,15 +615,15 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
       vx[q] = ux;
       vy[q] = uy;
       vz[q] = uz;
-      if (flags & kFlagHits) I.hits[p0 + q] = hits;
-      if (flags & kFlagKeepPerm) I.perm[p0 + q] = src;
+      if (flags & kFlagHits) gmem(I.hits)[p0 + q] = hits;
+      if (flags & kFlagKeepPerm) gmem(I.perm)[p0 + q] = src;
     }
-    count_keys(I.count, key == kDeferredKey ? kInvalidKey : key);
+    count_keys(gmem(I.count), key == kDeferredKey ? kInvalidKey : key);
     }
   }
   if (MODE == kModePhysics) {
     // queue the tile for the wall pass when any of its particles is near
-    if (__syncthreads_or(any_near) && tid == 0) I.near_tiles[atomicAdd(&I.ctr[5], 1u)] = tile;
+    if (__syncthreads_or(any_near) && tid == 0) gmem(I.near_tiles)[gatomic_add(gmem(I.ctr) + 5, 1u)] = tile;
   }
   // (6) sampling of the post-move velocities, one lane per cell
   if ((flags & kFlagSample) && I.sample_on) {
@@ -582,7 +634,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
       if (b > a) {
         if (MODE == kModeReference) {
           // SampleCellStatisticsDev (dsmc_kernel.cu:856-890), FP32 sums
-          float* S = reinterpret_cast<float*>(I.samp);
+          const gptr<float> S = gmem(reinterpret_cast<float*>(I.samp));
           const uint32_t C = I.ncells;
           float sn = S[c], sx = S[C + c], sy = S[2 * C + c], sz = S[3 * C + c], s2 = S[4 * C + c];
           for (uint32_t q = a; q < b; ++q) {
@@ -595,7 +647,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
           }
           S[c] = sn; S[C + c] = sx; S[2 * C + c] = sy; S[3 * C + c] = sz; S[4 * C + c] = s2;
         } else {
-          double* S = I.samp;
+          const gptr<double> S = gmem(I.samp);
           const uint32_t C = I.ncells;
           double sn = S[c], sx = S[C + c], sy = S[2 * C + c], sz = S[3 * C + c], s2 = S[4 * C + c];
           for (uint32_t q = a; q < b; ++q) {
@@ -617,8 +669,9 @@ template <uint32_t MODE>
 __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __restrict__ Ip, uint32_t par, uint32_t flags) {
   const InstDesc& I = Ip[blockIdx.y];
   if (blockIdx.x >= I.n_vwg) return;  // batched launches size the grid for the largest instance
-  __shared__ uint32_t s_start[kTileCells + 1];
-  __shared__ uint8_t s_nearc[kTileCells];
+  // cell offsets and near flags of the current tile and of the next one
+  __shared__ uint32_t s_start[2][kTileCells + 1];
+  __shared__ uint8_t s_nearc[2][kTileCells];
   __shared__ uint32_t s_raw[kLdsCap];
   __shared__ uint32_t s_perm[kLdsCap];
   __shared__ uint8_t s_owner[kLdsCap];
@@ -637,25 +690,45 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
   CollAcc cacc = {0.0, 0.0, 0.0};
   if (vw == 0 && tid == 0) I.ctr[0] = I.ctr[1];
 
-  for (TileWalk w = tile_walk_begin(I.n_vwg, vw);; w.e += w.step) {
-    const uint32_t j = tile_walk_pos(w);
-    if (j >= I.ntiles) break;
-    const uint32_t tile = I.tile_order ? I.tile_order[j] : j;
+  TileWalk w = tile_walk_begin(I.n_vwg, vw);
+  uint32_t j = tile_walk_pos(w);
+  uint32_t tile = kInvalidKey;
+  if (j < I.ntiles) {
+    tile = I.tile_order ? gmem(I.tile_order)[j] : j;
     const uint32_t c0 = tile * I.tile_cells;
     const uint32_t nc = min(I.tile_cells, I.ncells - c0);
-    s_start[tid] = I.cell_start[c0 + min(tid, nc)];
-    if (tid == 0) s_start[nc] = I.cell_start[c0 + nc];
-    if (MODE == kModePhysics) s_nearc[tid] = (I.near_cell && tid < nc) ? I.near_cell[c0 + tid] : (uint8_t)0;
-    __syncthreads();
-    const uint32_t p0 = s_start[0];
-    const uint32_t np = s_start[nc] - p0;
+    s_start[0][tid] = gmem(I.cell_start)[c0 + min(tid, nc)];
+    if (tid == 0) s_start[0][nc] = gmem(I.cell_start)[c0 + nc];
+    if (MODE == kModePhysics)
+      s_nearc[0][tid] = (I.near_cell && tid < nc) ? gmem(I.near_cell)[c0 + tid] : (uint8_t)0;
+  }
+  __syncthreads();
+  uint32_t pf[kPermPrefetch];
+  if (MODE == kModePhysics && tile != kInvalidKey) {
+    const uint32_t nc = min(I.tile_cells, I.ncells - tile * I.tile_cells);
+    const uint32_t p0 = s_start[0][0], np = s_start[0][nc] - p0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPermPrefetch; ++k) {
+      const uint32_t q = tid + k * kTileThreads;
+      pf[k] = (np <= kLdsCap && q < np) ? gmem(I.perm)[p0 + q] : 0u;
+    }
+  }
+  for (uint32_t b = 0; tile != kInvalidKey; b ^= 1u) {
+    w.e += w.step;
+    const uint32_t jn = tile_walk_pos(w);
+    const uint32_t tile_n = jn < I.ntiles ? (I.tile_order ? gmem(I.tile_order)[jn] : jn) : kInvalidKey;
+    const uint32_t c0 = tile * I.tile_cells;
+    const uint32_t nc = min(I.tile_cells, I.ncells - c0);
+    const uint32_t p0 = s_start[b][0];
+    const uint32_t np = s_start[b][nc] - p0;
     if (np <= kLdsCap)
-      tile_body<MODE, true>(I, par, flags, tile, c0, nc, p0, np, s_start, s_nearc, s_raw, s_perm, s_owner, s_vx,
-                            s_vy, s_vz, rng, cacc);
+      tile_body<MODE, true>(I, par, flags, tile, c0, nc, p0, np, s_start[b], s_nearc[b], tile_n, s_start[b ^ 1u],
+                            s_nearc[b ^ 1u], pf, s_raw, s_perm, s_owner, s_vx, s_vy, s_vz, rng, cacc);
     else
-      tile_body<MODE, false>(I, par, flags, tile, c0, nc, p0, np, s_start, s_nearc, s_raw, s_perm, s_owner, s_vx,
-                             s_vy, s_vz, rng, cacc);
+      tile_body<MODE, false>(I, par, flags, tile, c0, nc, p0, np, s_start[b], s_nearc[b], tile_n, s_start[b ^ 1u],
+                             s_nearc[b ^ 1u], pf, s_raw, s_perm, s_owner, s_vx, s_vy, s_vz, rng, cacc);
     __syncthreads();
+    tile = tile_n;
   }
   mt_store(I, stream, rng);
 
@@ -671,7 +744,7 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
   if (tid < kAccSlots) {
     double t = 0.0;
     for (uint32_t w = 0; w < kTileThreads / 64; ++w) t += s_red[w][tid];
-    I.wg_acc[vw * kAccSlots + tid] += t;
+    gmem(I.wg_acc)[vw * kAccSlots + tid] += t;
   }
 }
 
@@ -689,7 +762,10 @@ template __global__ void k_tile<kModePhysics>(const InstDesc*, uint32_t, uint32_
 // and added to the tile's own accumulator (deterministic for any schedule).
 constexpr uint32_t kWallScan = 1024;  // positions compacted per pass
 
-__global__ void __launch_bounds__(kTileThreads) k_walls(const InstDesc* __restrict__ Ip, uint32_t opar,
+#ifndef DSMC_WALLS_MINB
+#define DSMC_WALLS_MINB 1
+#endif
+__global__ void __launch_bounds__(kTileThreads, DSMC_WALLS_MINB) k_walls(const InstDesc* __restrict__ Ip, uint32_t opar,
                                                        uint32_t flags) {
   const InstDesc& I = Ip[blockIdx.y];
   if (!I.near_tiles) return;
@@ -699,19 +775,19 @@ __global__ void __launch_bounds__(kTileThreads) k_walls(const InstDesc* __restri
   const uint32_t ntl = I.ctr[5];
   const uint32_t stepno = I.ctr[4];
   const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / 64;
-  float* __restrict__ px = I.px[opar];
-  float* __restrict__ py = I.py[opar];
-  float* __restrict__ pz = I.pz[opar];
-  float* __restrict__ vx = I.vx[opar];
-  float* __restrict__ vy = I.vy[opar];
-  float* __restrict__ vz = I.vz[opar];
-  uint32_t* __restrict__ key = I.key[opar];
+  const gptr<float> px = gmem(I.px[opar]);
+  const gptr<float> py = gmem(I.py[opar]);
+  const gptr<float> pz = gmem(I.pz[opar]);
+  const gptr<float> vx = gmem(I.vx[opar]);
+  const gptr<float> vy = gmem(I.vy[opar]);
+  const gptr<float> vz = gmem(I.vz[opar]);
+  const gptr<uint32_t> key = gmem(I.key[opar]);
   for (uint32_t j = blockIdx.x; j < ntl; j += gridDim.x) {
-    const uint32_t tile = I.near_tiles[j];
+    const uint32_t tile = gmem(I.near_tiles)[j];
     const uint32_t c0 = tile * I.tile_cells;
     const uint32_t nc = min(I.tile_cells, I.ncells - c0);
-    const uint32_t p0 = I.cell_start[c0];
-    const uint32_t np = I.cell_start[c0 + nc] - p0;
+    const uint32_t p0 = gmem(I.cell_start)[c0];
+    const uint32_t np = gmem(I.cell_start)[c0 + nc] - p0;
     WallAcc acc = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t b0 = 0; b0 < np; b0 += kWallScan) {
       const uint32_t bn = min(kWallScan, np - b0);
@@ -744,8 +820,8 @@ __global__ void __launch_bounds__(kTileThreads) k_walls(const InstDesc* __restri
         vy[slot] = uy;
         vz[slot] = uz;
         key[slot] = k;
-        if (k != kInvalidKey) atomicAdd(&I.count[k], 1u);
-        if (flags & kFlagHits) I.hits[slot] = hits;
+        if (k != kInvalidKey) gatomic_add(gmem(I.count) + k, 1u);
+        if (flags & kFlagHits) gmem(I.hits)[slot] = hits;
       }
       __syncthreads();
     }
@@ -760,7 +836,7 @@ __global__ void __launch_bounds__(kTileThreads) k_walls(const InstDesc* __restri
     if (tid < 4) {
       double t = 0.0;
       for (uint32_t w = 0; w < kTileThreads / 64; ++w) t += s_red[w][tid];
-      I.tile_acc[(size_t)tile * 4 + tid] += t;
+      gmem(I.tile_acc)[(size_t)tile * 4 + tid] += t;
     }
     __syncthreads();
   }
