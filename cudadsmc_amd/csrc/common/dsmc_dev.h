@@ -123,6 +123,7 @@ struct InstDesc {
                         // [5] tiles queued for the wall pass
   const uint32_t* tile_order;  // tiles in locality order [ntiles] (null: identity)
   uint32_t* near_tiles; // PHYSICS with a body: tiles with deferred particles this step [ntiles]
+  unsigned long long* defer_bits;  // PHYSICS with a body: bit s set = slot s deferred [cap / 64 + 2]
   double* tile_acc;     // PHYSICS with a body: wall {Fx, Fy, Fz, heat} sums per tile [ntiles][4]
   uint32_t wall_key[2]; // Philox key of the wall re-emission draws
   double* wg_acc;       // [n_vwg][kAccSlots]

@@ -619,6 +619,19 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
       if (flags & kFlagKeepPerm) gmem(I.perm)[p0 + q] = src;
     }
     count_keys(gmem(I.count), key == kDeferredKey ? kInvalidKey : key);
+    if (MODE == kModePhysics) {
+      // mark the wave's deferred slots in the bitmask the wall pass reads
+      const uint64_t m = __ballot(key == kDeferredKey);
+      if (m && lane_id() == 0) {
+        const uint32_t s0 = p0 + (r0 + k) * kTileThreads + (tid & ~63u);
+        const uint32_t sh = s0 & 63u;
+        const gptr<unsigned long long> wb = gmem(I.defer_bits) + s0 / 64u;
+        __hip_atomic_fetch_or(wb, (unsigned long long)(m << sh), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (sh)
+          __hip_atomic_fetch_or(wb + 1, (unsigned long long)(m >> (64u - sh)), __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
     }
   }
   if (MODE == kModePhysics) {
@@ -762,10 +775,7 @@ template __global__ void k_tile<kModePhysics>(const InstDesc*, uint32_t, uint32_
 // and added to the tile's own accumulator (deterministic for any schedule).
 constexpr uint32_t kWallScan = 1024;  // positions compacted per pass
 
-#ifndef DSMC_WALLS_MINB
-#define DSMC_WALLS_MINB 1
-#endif
-__global__ void __launch_bounds__(kTileThreads, DSMC_WALLS_MINB) k_walls(const InstDesc* __restrict__ Ip, uint32_t opar,
+__global__ void __launch_bounds__(kTileThreads) k_walls(const InstDesc* __restrict__ Ip, uint32_t opar,
                                                        uint32_t flags) {
   const InstDesc& I = Ip[blockIdx.y];
   if (!I.near_tiles) return;
@@ -790,23 +800,49 @@ __global__ void __launch_bounds__(kTileThreads, DSMC_WALLS_MINB) k_walls(const I
     const uint32_t np = gmem(I.cell_start)[c0 + nc] - p0;
     WallAcc acc = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t b0 = 0; b0 < np; b0 += kWallScan) {
+      // deferred slots of [p0 + b0, p0 + b0 + bn) from the bitmask words
+      // covering it (at most 17), in increasing order; the words' bits in
+      // this range are cleared for the next step (bits outside it belong to
+      // neighbouring tiles)
       const uint32_t bn = min(kWallScan, np - b0);
-      uint32_t cnt = 0;
-      for (uint32_t r = 0; r < bn; r += kTileThreads) {
-        const uint32_t q = b0 + r + tid;
-        const bool d = (r + tid < bn) && key[p0 + q] == kDeferredKey;
-        const uint64_t m = __ballot(d);
-        if (lane == 0) s_cnt[wid] = (uint32_t)__popcll(m);
-        __syncthreads();
-        uint32_t before = cnt, total = 0;
-        for (uint32_t w = 0; w < kTileThreads / 64; ++w) {
-          if (w < wid) before += s_cnt[w];
-          total += s_cnt[w];
+      const uint32_t s_lo = p0 + b0, s_hi = s_lo + bn;
+      const uint32_t w_lo = s_lo / 64u, nw = (s_hi - 1u) / 64u - w_lo + 1u;
+      uint64_t word = 0;
+      uint32_t before = 0;
+      if (wid == 0) {
+        uint64_t in_range = 0;
+        if (lane < nw) {
+          const uint32_t base = (w_lo + lane) * 64u;
+          in_range = ~0ull;
+          if (base < s_lo) in_range &= ~0ull << (s_lo - base);
+          if (s_hi - base < 64u) in_range &= (1ull << (s_hi - base)) - 1ull;
+          word = gmem(I.defer_bits)[w_lo + lane] & in_range;
         }
-        if (d) s_q[before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = q;
-        cnt += total;
-        __syncthreads();
+        // exclusive prefix of the popcounts over the words
+        const uint32_t pc = (uint32_t)__popcll(word);
+        uint32_t incl = pc;
+        for (uint32_t off = 1; off < 32u; off <<= 1) {
+          const uint32_t t = __shfl_up(incl, off);
+          if (lane >= off) incl += t;
+        }
+        before = incl - pc;
+        if (lane == 31) s_cnt[0] = incl;
+        if (lane < nw && word) {
+          uint64_t m = word;
+          uint32_t k = before;
+          const uint32_t base = (w_lo + lane) * 64u;
+          while (m) {
+            const uint32_t bit = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+            s_q[k++] = base + bit - p0;
+            m &= m - 1ull;
+          }
+        }
+        if (lane < nw)
+          __hip_atomic_fetch_and(gmem(I.defer_bits) + w_lo + lane, ~in_range, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
       }
+      __syncthreads();
+      const uint32_t cnt = s_cnt[0];
       for (uint32_t i = tid; i < cnt; i += kTileThreads) {
         const uint32_t slot = p0 + s_q[i];
         float x = px[slot], y = py[slot], z = pz[slot];

@@ -587,6 +587,8 @@ static int finish_loading(dsmc_solver* h, uint32_t n, uint32_t* n_out) {
   if (h->physics() && I.has_geom && !I.near_tiles) {
     // wall pass queue and per-tile wall force sums (tile count is fixed now)
     CHECK_RC(h->alloc(&I.near_tiles, I.ntiles));
+    CHECK_RC(h->alloc(&I.defer_bits, (size_t)I.cap / 64 + 2));
+    HIPC(hipMemsetAsync(I.defer_bits, 0, ((size_t)I.cap / 64 + 2) * 8, st));
     CHECK_RC(h->alloc(&I.tile_acc, (size_t)I.ntiles * 4));
     HIPC(hipMemsetAsync(I.tile_acc, 0, (size_t)I.ntiles * 4 * sizeof(double), st));
     CHECK_RC(h->upload_desc());
