@@ -545,6 +545,8 @@ struct CollAcc {
 
 // NTC with N(N-1)/2 candidate pairs, distinct partners and the VHS
 // cross-section sigma*c_r = vhs_k * c_r^(2-2 omega) (Bird 1994, sec. 4.3).
+// Partners come from the high word of draw * range (no integer divide) and
+// a pair collides when sigma*c_r > R * (sigma*c_r)max (no float divide).
 __device__ __forceinline__ void ntc_physics(const InstDesc& I, float* vx, float* vy, float* vz, uint32_t n,
                                             float vol, float2& nt, Mt& rng, CollAcc& acc) {
   if (n < 2u) return;
@@ -558,14 +560,14 @@ __device__ __forceinline__ void ntc_physics(const InstDesc& I, float* vx, float*
   }
   acc.sel += (double)pairs;
   for (uint32_t k = 0; k < pairs; ++k) {
-    uint32_t i = mt_next(rng) % n;
-    uint32_t j = mt_next(rng) % (n - 1u);
+    uint32_t i = __umulhi(mt_next(rng), n);
+    uint32_t j = __umulhi(mt_next(rng), n - 1u);
     if (j >= i) ++j;
     float dx = vx[i] - vx[j], dy = vy[i] - vy[j], dz = vz[i] - vz[j];
     float mag = sqrtf(dx * dx + dy * dy + dz * dz);
     float sc = (I.vhs_exp == 1.0f) ? I.vhs_k * mag : (mag > 0.0f ? I.vhs_k * dm_powf(mag, I.vhs_exp) : 0.0f);
     if (sc > nt.x) nt.x = sc;
-    if (sc / nt.x > mt_float(rng)) {
+    if (sc > mt_float(rng) * nt.x) {
       acc.coll += 1.0;
       acc.cr += (double)mag;
       scatter_pair(vx, vy, vz, i, j, mag, rng);

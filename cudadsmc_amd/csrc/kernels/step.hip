@@ -470,9 +470,17 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
     uint32_t c = own[q];
     uint32_t a = s_start[c] - p0, b = s_start[c + 1] - p0;
     uint32_t v = raw[q];
-    uint32_t r = 0;
-    for (uint32_t j = a; j < b; ++j) r += raw[j] < v ? 1u : 0u;
-    srt[a + r] = v;
+    // four independent reads per iteration: dense cells are long loops
+    uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+    uint32_t j = a;
+    for (; j + 4u <= b; j += 4u) {
+      r0 += raw[j] < v ? 1u : 0u;
+      r1 += raw[j + 1u] < v ? 1u : 0u;
+      r2 += raw[j + 2u] < v ? 1u : 0u;
+      r3 += raw[j + 3u] < v ? 1u : 0u;
+    }
+    for (; j < b; ++j) r0 += raw[j] < v ? 1u : 0u;
+    srt[a + r0 + r1 + r2 + r3] = v;
   }
   __syncthreads();
   // the collision lane's cell state, loaded now so that its latency overlaps

@@ -536,14 +536,15 @@ struct oracle_sim {
     pairs = std::min(pairs, kMaxPairs);
     a[0] += pairs;
     for (uint32_t k = 0; k < pairs; ++k) {
-      uint32_t i = r.next() % n;
-      uint32_t j = r.next() % (n - 1);
+      // partner = high word of draw * range; collide when sc > R * max
+      uint32_t i = (uint32_t)(((uint64_t)r.next() * n) >> 32);
+      uint32_t j = (uint32_t)(((uint64_t)r.next() * (n - 1)) >> 32);
       if (j >= i) ++j;
       float dx = ux[i] - ux[j], dy = uy[i] - uy[j], dz = uz[i] - uz[j];
       float mag = std::sqrt(dx * dx + dy * dy + dz * dz);
       float sc = vhs_exp == 1.0f ? vhs_k * mag : (mag > 0.0f ? vhs_k * dm_powf(mag, vhs_exp) : 0.0f);
       if (sc > nt[0]) nt[0] = sc;
-      if (sc / nt[0] > r.uniform()) {
+      if (sc > r.uniform() * nt[0]) {
         a[1] += 1.0;
         a[2] += mag;
         scatter(ux, uy, uz, i, j, mag, r);
