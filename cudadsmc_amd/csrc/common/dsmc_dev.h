@@ -4,7 +4,8 @@
 //   particles  x,y,z,vx,vy,vz : float[cap], double-buffered (parity = step & 1)
 //   keys                      : uint32[cap] per buffer (cell of the particle,
 //                               0xFFFFFFFF = removed)
-//   count / cursor            : uint32[ncells]      next-step counting, scatter cursors
+//   count                     : uint32[ncells]      next-step counting
+//   rank                      : uint32[cap]         place of each stored particle in its cell's count
 //   cell_start                : uint32[ncells + 1]  offsets of the current order
 //   perm / perm_tmp           : uint32[cap]         sorted position -> storage index
 //   ntc                       : float2[ncells]      {(sigma c_r)max, remainder}
@@ -108,9 +109,9 @@ struct InstDesc {
   uint32_t cap;
   // ---- cells
   uint32_t* count;
-  uint32_t* cursor;
   uint32_t* cell_start;
   uint32_t* perm;
+  uint32_t* rank;       // per stored slot: its place among the particles counted into its cell [cap]
   uint32_t* perm_tmp;   // overflow tiles: sorted copy of the segment
   uint8_t* owner_tmp;   // overflow tiles: cell of each position within the tile
   float2* ntc;

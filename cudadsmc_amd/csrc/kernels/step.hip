@@ -8,9 +8,10 @@
 //  * Keys of the next step are computed by the move itself and counted per
 //    cell with wave-aggregated atomics, so the "sort" is a counting sort:
 //    an exclusive scan of the counts gives the cell offsets
-//    (cellStartEnd of FindCellBoundariesDev, dsmc_kernel.cu:1734-1787) and a
-//    scatter with per-cell cursors gives the permutation.
-//  * The tile kernel owns 256 consecutive cells: it restores the stable
+//    (cellStartEnd of FindCellBoundariesDev, dsmc_kernel.cu:1734-1787). The
+//    counting atomics return each particle's rank in its cell, which is
+//    kept, so the scatter that builds the permutation needs no atomics.
+//  * The tile kernel owns a tile of consecutive cells: it restores the stable
 //    order inside each cell segment (ascending storage index, i.e. the
 //    permutation of a stable sort_by_key, dsmc_kernel.cu:2048), gathers the
 //    particles (the REORDER gather, :1777-1786), runs the NTC loop one lane
@@ -47,6 +48,41 @@ __device__ __forceinline__ void mt_load(const InstDesc& I, uint32_t s, Mt& m) {
 __device__ __forceinline__ void mt_store(const InstDesc& I, uint32_t s, const Mt& m) {
 #pragma unroll
   for (int w = 0; w < kMtN; ++w) gmem(I.mt_state)[(uint32_t)w * I.n_streams + s] = m.st[w];
+}
+
+// Wave-aggregated atomicAdd(&cnt[key], 1) for keys of consecutive lanes and
+// the lane's rank among the particles counted into its cell (the count
+// before it): one atomic per run of equal keys. Invalid keys are skipped
+// (rank 0).
+// Split in two so that callers can issue the atomics of several rounds
+// before waiting for any returned count: count_issue returns the head's
+// count (0 elsewhere) and the wave's run heads, count_finish the rank.
+__device__ __forceinline__ uint32_t count_issue(gptr<uint32_t> cnt, uint32_t key, uint64_t& heads) {
+  const uint32_t lane = lane_id();
+  const uint32_t prev = __shfl_up(key, 1);
+  const bool head = (lane == 0) || (key != prev);
+  heads = __ballot(head);
+  uint32_t base = 0;
+  if (head && key != kInvalidKey) {
+    const uint64_t below_incl = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const uint64_t above = heads & ~below_incl;
+    const uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
+    base = gatomic_add(cnt + key, next - lane);
+  }
+  return base;
+}
+
+__device__ __forceinline__ uint32_t count_finish(uint32_t base, uint64_t heads) {
+  const uint32_t lane = lane_id();
+  const uint64_t below_incl = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const uint32_t hl = 63u - (uint32_t)__clzll((long long)(heads & below_incl));
+  return __shfl(base, (int)hl) + (lane - hl);
+}
+
+__device__ __forceinline__ uint32_t count_rank(gptr<uint32_t> cnt, uint32_t key) {
+  uint64_t heads;
+  const uint32_t base = count_issue(cnt, key, heads);
+  return count_finish(base, heads);
 }
 
 // Wave-aggregated atomicAdd(&cnt[key], 1) for keys of consecutive lanes:
@@ -111,7 +147,8 @@ __global__ void k_keys_count(const InstDesc* __restrict__ Ip, uint32_t par, uint
     key = I.mode == kModePhysics ? cell_key_phys(I, x, y, z) : cell_key(I, x, y, z);
     I.key[par][i] = key;
   }
-  count_keys(gmem(I.count), key);
+  const uint32_t rk = count_rank(gmem(I.count), key);
+  if (i < n) gmem(I.rank)[i] = rk;
 }
 
 // Plain counting of arbitrary keys (sort operator).
@@ -221,7 +258,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(uint32_t* in, uint3
 }
 
 // Step scans over a batch of instances (grid.y = instance). which = 0: the
-// cell counts (count -> cell_start, cursor, count cleared, total in ctr[1]);
+// cell counts (count -> cell_start, count cleared, total in ctr[1]);
 // which = 1: the injection face counts, in place (total in face_count[n]).
 // scratch: kScanMaxTiles + 1 words per instance.
 __device__ __forceinline__ void bscan_args(const InstDesc& I, uint32_t which, uint32_t*& in, uint32_t& n) {
@@ -299,7 +336,6 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __
   if (blockIdx.x * kScanTile >= n) return;
   const uint32_t* ts = scratch + blockIdx.y * (dsmc_launch::kScanMaxTiles + 1);
   uint32_t* out = which ? I.face_count : I.cell_start;
-  uint32_t* cursor = which ? nullptr : I.cursor;
   __shared__ uint32_t sh[kScanThreads / 64 + 1];
   uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
@@ -317,7 +353,6 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __
     uint32_t i = base + k;
     if (i < n) {
       out[i] = off;
-      if (cursor) cursor[i] = off;
       if (which == 0) in[i] = 0u;
     }
     off += v[k];
@@ -326,48 +361,22 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __
 }
 
 // =================================================================== scatter
-// perm[cursor[key[i]]++] = i with one atomic per run of equal keys in a wave.
-// Each thread takes kScatterBatch positions 256 apart per pass: their key
-// loads, then their atomics, are issued together so that the latencies of
-// the returning atomics overlap.
-constexpr uint32_t kScatterBatch = 4;
-
+// perm[cell_start[key[i]] + rank[i]] = i: the rank of each stored particle
+// among those counted into its cell was kept by the kernel that counted it,
+// so no atomics are needed here. The order within a cell is the counting
+// order; the tile kernel sorts each cell's segment by storage index.
 __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip, uint32_t par) {
   const InstDesc& I = Ip[blockIdx.y];
   const uint32_t n = I.ctr[0];
   const gptr<const uint32_t> keys = gmem<const uint32_t>(I.key[par]);
-  const gptr<uint32_t> cursor = gmem(I.cursor);
+  const gptr<const uint32_t> rank = gmem<const uint32_t>(I.rank);
+  const gptr<const uint32_t> start = gmem<const uint32_t>(I.cell_start);
   const gptr<uint32_t> perm = gmem(I.perm);
-  const uint32_t lane = lane_id();
-  const uint64_t below_incl = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-  for (uint32_t base = blockIdx.x * (256u * kScatterBatch); base < n; base += gridDim.x * (256u * kScatterBatch)) {
-    uint32_t key[kScatterBatch], slot[kScatterBatch];
-    uint64_t heads[kScatterBatch];
-#pragma unroll
-    for (uint32_t k = 0; k < kScatterBatch; ++k) {
-      const uint32_t i = base + k * 256u + threadIdx.x;
-      key[k] = i < n ? keys[i] : kInvalidKey;
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < kScatterBatch; ++k) {
-      const uint32_t prev = __shfl_up(key[k], 1);
-      const bool head = (lane == 0) || (key[k] != prev);
-      heads[k] = __ballot(head);
-      slot[k] = 0;
-      if (head && key[k] != kInvalidKey) {
-        uint64_t above = heads[k] & ~below_incl;
-        uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
-        slot[k] = gatomic_add(cursor + key[k], next - lane);
-      }
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < kScatterBatch; ++k) {
-      const uint32_t i = base + k * 256u + threadIdx.x;
-      uint32_t hl = 63u - (uint32_t)__clzll((long long)(heads[k] & below_incl));
-      uint32_t hslot = __shfl(slot[k], (int)hl);
-      uint32_t dst = hslot + (lane - hl);
-      if (key[k] != kInvalidKey && dst < I.cap) perm[dst] = i;
-    }
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const uint32_t key = keys[i];
+    if (key == kInvalidKey) continue;
+    const uint32_t dst = start[key] + rank[i];
+    if (dst < I.cap) perm[dst] = i;
   }
 }
 
@@ -579,6 +588,9 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
     // positions of up to kMoveBatch rounds in flight before the first store
     float bx[kMoveBatch], by[kMoveBatch], bz[kMoveBatch];
     uint32_t bsrc[kMoveBatch];
+    // counting atomics of the batch: ranks read after the last round
+    uint32_t cbase[kMoveBatch];
+    uint64_t cheads[kMoveBatch];
 #pragma unroll
     for (uint32_t k = 0; k < kMoveBatch; ++k) {
       const uint32_t q = (r0 + k) * kTileThreads + tid;
@@ -626,7 +638,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
       if (flags & kFlagHits) gmem(I.hits)[p0 + q] = hits;
       if (flags & kFlagKeepPerm) gmem(I.perm)[p0 + q] = src;
     }
-    count_keys(gmem(I.count), key == kDeferredKey ? kInvalidKey : key);
+    cbase[k] = count_issue(gmem(I.count), key == kDeferredKey ? kInvalidKey : key, cheads[k]);
     if (MODE == kModePhysics) {
       // mark the wave's deferred slots in the bitmask the wall pass reads
       const uint64_t m = __ballot(key == kDeferredKey);
@@ -640,6 +652,13 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
                                 __HIP_MEMORY_SCOPE_AGENT);
       }
     }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kMoveBatch; ++k) {
+      if (r0 + k >= rounds) break;  // uniform over the workgroup
+      const uint32_t q = (r0 + k) * kTileThreads + tid;
+      const uint32_t rk = count_finish(cbase[k], cheads[k]);
+      if (q < np) gmem(I.rank)[p0 + q] = rk;
     }
   }
   if (MODE == kModePhysics) {
@@ -864,7 +883,7 @@ __global__ void __launch_bounds__(kTileThreads) k_walls(const InstDesc* __restri
         vy[slot] = uy;
         vz[slot] = uz;
         key[slot] = k;
-        if (k != kInvalidKey) gatomic_add(gmem(I.count) + k, 1u);
+        if (k != kInvalidKey) gmem(I.rank)[slot] = gatomic_add(gmem(I.count) + k, 1u);
         if (flags & kFlagHits) gmem(I.hits)[slot] = hits;
       }
       __syncthreads();
@@ -1140,7 +1159,7 @@ __global__ void k_inject_generate(const InstDesc* __restrict__ Ip, uint32_t opar
       I.vx[opar][idx] = v[0]; I.vy[opar][idx] = v[1]; I.vz[opar][idx] = v[2];
       uint32_t key = cell_key_phys(I, x, y, zz);
       I.key[opar][idx] = key;
-      atomicAdd(&I.count[key], 1u);
+      gmem(I.rank)[idx] = gatomic_add(gmem(I.count) + key, 1u);
     }
   }
   mt_store(I, s, rng);

@@ -323,11 +323,11 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
     CHECK_RC(h->alloc(&I.key[b], cap));
   }
   CHECK_RC(h->alloc(&I.perm, cap));
+  CHECK_RC(h->alloc(&I.rank, cap));
   CHECK_RC(h->alloc(&I.perm_tmp, cap));
   CHECK_RC(h->alloc(&I.owner_tmp, cap));
   CHECK_RC(h->alloc(&I.hits, cap));
   CHECK_RC(h->alloc(&I.count, C));
-  CHECK_RC(h->alloc(&I.cursor, C));
   CHECK_RC(h->alloc(&I.cell_start, (size_t)C + 1));
   CHECK_RC(h->alloc(&I.ntc, C));
   CHECK_RC(h->alloc(&I.cstat, C));
@@ -414,8 +414,8 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
     HIPC(hipMemcpyAsync(I.taus, ts.data(), ts.size() * 4, hipMemcpyHostToDevice, st));
   }
   HIPC(hipStreamSynchronize(st));
-  // k_scatter covers 1024 positions per workgroup and pass
-  h->scatter_grid = std::min<uint32_t>((cap + 1023) / 1024, 4096u);
+  // grid-stride scatter: 256 positions per workgroup and pass
+  h->scatter_grid = std::min<uint32_t>((cap + 255) / 256, 4096u);
   if (phys) {
     // F_n and the inflow per face for the whole box; dsmc_set_geometry
     // redoes both for the fluid volume
