@@ -121,11 +121,16 @@ struct InstDesc {
   const uint8_t* near_cell;  // PHYSICS with a body: cells within one cell of a collision-grid cell in use
   // ---- scalars and per-workgroup accumulators
   uint32_t* ctr;        // [0] stored, [1] sorted, [2] injected particles, [3] status bits, [4] step number,
-                        // [5] tiles queued for the wall pass
+                        // [5] deferred particles of the step
   const uint32_t* tile_order;  // tiles in locality order [ntiles] (null: identity)
-  uint32_t* near_tiles; // PHYSICS with a body: tiles with deferred particles this step [ntiles]
   unsigned long long* defer_bits;  // PHYSICS with a body: bit s set = slot s deferred [cap / 64 + 2]
-  double* tile_acc;     // PHYSICS with a body: wall {Fx, Fy, Fz, heat} sums per tile [ntiles][4]
+  uint32_t* defer_list; // PHYSICS with a body: deferred slots of this step, compacted [cap]
+  uint32_t n_wall_wg;   // PHYSICS with a body: workgroups of the wall pass
+  unsigned long long* wall_fix;  // [4] fixed-point wall sums of the step, [4] workgroups done
+  double wall_scale[2]; // fixed-point scale of a velocity change (2^36 / v_ref) and of a change of |v|^2
+                        // (2^33 / v_ref^2)
+  double wall_unit[2];  // value of one fixed-point unit: m v_ref / 2^36 (momentum), m v_ref^2 / 2^34 (energy)
+  double* wall_acc;     // PHYSICS with a body: wall {Fx, Fy, Fz, heat} sums over the steps [4]
   uint32_t wall_key[2]; // Philox key of the wall re-emission draws
   double* wg_acc;       // [n_vwg][kAccSlots]
   uint32_t* hits;       // per sorted particle wall hits (debug, may be null)

@@ -383,8 +383,12 @@ __device__ __forceinline__ bool phys_face(float& q, float& v, float lo, float hi
   return true;
 }
 
+// Wall sums in 64-bit fixed point (momentum change in units of
+// m v_ref / 2^36, energy change in units of m v_ref^2 / 2^34): integer
+// addition does not depend on the order of the terms, so the sums are the
+// same for any schedule of the wall pass.
 struct WallAcc {
-  double fx, fy, fz, heat;
+  long long fx, fy, fz, heat;
 };
 
 // PHYSICS move. A particle is "near" when its cell is flagged as within one
@@ -461,12 +465,11 @@ __device__ __forceinline__ uint32_t move_walls(const InstDesc& I, float& px, flo
       uy = vy - 2.0f * dd * n.y;
       uz = vz - 2.0f * dd * n.z;
     }
-    double m = (double)I.mass;
-    acc.fx += m * ((double)vx - (double)ux);
-    acc.fy += m * ((double)vy - (double)uy);
-    acc.fz += m * ((double)vz - (double)uz);
-    acc.heat += 0.5 * m * (((double)vx * vx + (double)vy * vy + (double)vz * vz) -
-                           ((double)ux * ux + (double)uy * uy + (double)uz * uz));
+    acc.fx += __double2ll_rn(((double)vx - (double)ux) * I.wall_scale[0]);
+    acc.fy += __double2ll_rn(((double)vy - (double)uy) * I.wall_scale[0]);
+    acc.fz += __double2ll_rn(((double)vz - (double)uz) * I.wall_scale[0]);
+    acc.heat += __double2ll_rn((((double)vx * vx + (double)vy * vy + (double)vz * vz) -
+                                ((double)ux * ux + (double)uy * uy + (double)uz * uz)) * I.wall_scale[1]);
     ++hits;
     tl = tl - s / sp;
     const float off = 1.0e-6f;

@@ -322,7 +322,7 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_tiles(const InstDesc* __
     if (which == 0) {
       I.ctr[1] = total;
       I.ctr[4] += 1u;  // step number (counter of the wall draws)
-      I.ctr[5] = 0u;   // tiles queued for the wall pass
+      I.ctr[5] = 0u;   // deferred particles of the step
     }
   }
 }
@@ -583,7 +583,6 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
   // (5) move, walls, box faces, next key; rounds keep waves converged for
   // the aggregated counting
   const uint32_t rounds = (np + kTileThreads - 1u) / kTileThreads;
-  int any_near = 0;
   for (uint32_t r0 = 0; r0 < rounds; r0 += kMoveBatch) {
     // positions of up to kMoveBatch rounds in flight before the first store
     float bx[kMoveBatch], by[kMoveBatch], bz[kMoveBatch];
@@ -619,7 +618,6 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
         key = cell_key(I, x, y, z);
       } else if (phys_near(I, s_nearc[own[q]] != 0, ux, uy, uz, I.dt)) {
         key = kDeferredKey;  // stored as is; k_walls moves it and counts its key
-        any_near = 1;
       } else {
         key = move_free(I, x, y, z, ux, uy, uz, I.dt);
       }
@@ -661,10 +659,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
       if (q < np) gmem(I.rank)[p0 + q] = rk;
     }
   }
-  if (MODE == kModePhysics) {
-    // queue the tile for the wall pass when any of its particles is near
-    if (__syncthreads_or(any_near) && tid == 0) gmem(I.near_tiles)[gatomic_add(gmem(I.ctr) + 5, 1u)] = tile;
-  }
+
   // (6) sampling of the post-move velocities, one lane per cell
   if ((flags & kFlagSample) && I.sample_on) {
     __syncthreads();
@@ -793,23 +788,54 @@ template __global__ void k_tile<kModePhysics>(const InstDesc*, uint32_t, uint32_
 
 // =============================================================== wall pass
 // PHYSICS: the particles the tile kernel deferred (key kDeferredKey, state
-// after collisions, before the move) fly with walls and faces here. One
-// workgroup per queued tile at a time: the deferred positions of the tile
-// are compacted in increasing order in LDS and shared out to all 256 lanes,
-// so the divergent traversal runs on full waves; the kernel holds no RNG
-// state (Philox draws) and little else, and runs at high occupancy. Wall
-// forces are summed per thread in a fixed order, reduced in a fixed order
-// and added to the tile's own accumulator (deterministic for any schedule).
-constexpr uint32_t kWallScan = 1024;  // positions compacted per pass
+// after collisions, before the move; one bit per slot in defer_bits) fly
+// with walls and faces here, in two launches.
+//
+// k_defer_list compacts the marked slots: each wave takes chunks of 64
+// bitmask words (4096 slots), finds their set bits with one coalesced load,
+// reserves room for them in defer_list with one atomic, writes them and
+// clears the words.
+__global__ void __launch_bounds__(kTileThreads) k_defer_list(const InstDesc* __restrict__ Ip) {
+  const InstDesc& I = Ip[blockIdx.y];
+  if (!I.defer_bits || blockIdx.x >= I.n_wall_wg) return;
+  const uint32_t lane = lane_id();
+  const uint32_t nwords = (I.ctr[1] + 63u) / 64u;  // deferred slots lie below the sorted count
+  const uint32_t nchunks = (nwords + 63u) / 64u;
+  const gptr<unsigned long long> bits = gmem(I.defer_bits);
+  const gptr<uint32_t> list = gmem(I.defer_list);
+  const uint32_t waves = I.n_wall_wg * (kTileThreads / 64u);
+  for (uint32_t ch = blockIdx.x * (kTileThreads / 64u) + threadIdx.x / 64u; ch < nchunks; ch += waves) {
+    const uint32_t w = ch * 64u + lane;
+    const unsigned long long word = w < nwords ? bits[w] : 0ull;
+    const uint32_t pc = (uint32_t)__popcll(word);
+    uint32_t incl = pc;
+    for (uint32_t off = 1; off < 64u; off <<= 1) {
+      const uint32_t t = __shfl_up(incl, off);
+      if (lane >= off) incl += t;
+    }
+    const uint32_t total = __shfl(incl, 63);
+    if (total == 0u) continue;  // uniform over the wave
+    uint32_t base = 0u;
+    if (lane == 0) base = gatomic_add(gmem(I.ctr) + 5, total);
+    uint32_t k = __shfl(base, 0) + incl - pc;
+    for (unsigned long long m = word; m; m &= m - 1ull) list[k++] = w * 64u + (uint32_t)__ffsll(m) - 1u;
+    if (word) bits[w] = 0ull;
+  }
+}
 
+// k_walls moves the listed particles, one per thread (a grid-stride loop
+// over the list). The pass holds no RNG state (Philox draws keyed by slot
+// and step). Wall forces and heat are summed in 64-bit fixed point, so the
+// order in which particles are listed and run does not change the sums;
+// the last workgroup to finish converts the step's sums and adds them to
+// wall_acc.
 __global__ void __launch_bounds__(kTileThreads) k_walls(const InstDesc* __restrict__ Ip, uint32_t opar,
                                                        uint32_t flags) {
   const InstDesc& I = Ip[blockIdx.y];
-  if (!I.near_tiles) return;
-  __shared__ uint32_t s_q[kWallScan];
-  __shared__ uint32_t s_cnt[kTileThreads / 64];
-  __shared__ double s_red[kTileThreads / 64][4];
-  const uint32_t ntl = I.ctr[5];
+  if (!I.defer_bits || blockIdx.x >= I.n_wall_wg) return;
+  __shared__ long long s_red[kTileThreads / 64][4];
+  __shared__ uint32_t s_last;
+  const uint32_t cnt = I.ctr[5];
   const uint32_t stepno = I.ctr[4];
   const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / 64;
   const gptr<float> px = gmem(I.px[opar]);
@@ -819,89 +845,61 @@ __global__ void __launch_bounds__(kTileThreads) k_walls(const InstDesc* __restri
   const gptr<float> vy = gmem(I.vy[opar]);
   const gptr<float> vz = gmem(I.vz[opar]);
   const gptr<uint32_t> key = gmem(I.key[opar]);
-  for (uint32_t j = blockIdx.x; j < ntl; j += gridDim.x) {
-    const uint32_t tile = gmem(I.near_tiles)[j];
-    const uint32_t c0 = tile * I.tile_cells;
-    const uint32_t nc = min(I.tile_cells, I.ncells - c0);
-    const uint32_t p0 = gmem(I.cell_start)[c0];
-    const uint32_t np = gmem(I.cell_start)[c0 + nc] - p0;
-    WallAcc acc = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t b0 = 0; b0 < np; b0 += kWallScan) {
-      // deferred slots of [p0 + b0, p0 + b0 + bn) from the bitmask words
-      // covering it (at most 17), in increasing order; the words' bits in
-      // this range are cleared for the next step (bits outside it belong to
-      // neighbouring tiles)
-      const uint32_t bn = min(kWallScan, np - b0);
-      const uint32_t s_lo = p0 + b0, s_hi = s_lo + bn;
-      const uint32_t w_lo = s_lo / 64u, nw = (s_hi - 1u) / 64u - w_lo + 1u;
-      uint64_t word = 0;
-      uint32_t before = 0;
-      if (wid == 0) {
-        uint64_t in_range = 0;
-        if (lane < nw) {
-          const uint32_t base = (w_lo + lane) * 64u;
-          in_range = ~0ull;
-          if (base < s_lo) in_range &= ~0ull << (s_lo - base);
-          if (s_hi - base < 64u) in_range &= (1ull << (s_hi - base)) - 1ull;
-          word = gmem(I.defer_bits)[w_lo + lane] & in_range;
-        }
-        // exclusive prefix of the popcounts over the words
-        const uint32_t pc = (uint32_t)__popcll(word);
-        uint32_t incl = pc;
-        for (uint32_t off = 1; off < 32u; off <<= 1) {
-          const uint32_t t = __shfl_up(incl, off);
-          if (lane >= off) incl += t;
-        }
-        before = incl - pc;
-        if (lane == 31) s_cnt[0] = incl;
-        if (lane < nw && word) {
-          uint64_t m = word;
-          uint32_t k = before;
-          const uint32_t base = (w_lo + lane) * 64u;
-          while (m) {
-            const uint32_t bit = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-            s_q[k++] = base + bit - p0;
-            m &= m - 1ull;
-          }
-        }
-        if (lane < nw)
-          __hip_atomic_fetch_and(gmem(I.defer_bits) + w_lo + lane, ~in_range, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
-      }
-      __syncthreads();
-      const uint32_t cnt = s_cnt[0];
-      for (uint32_t i = tid; i < cnt; i += kTileThreads) {
-        const uint32_t slot = p0 + s_q[i];
-        float x = px[slot], y = py[slot], z = pz[slot];
-        float ux = vx[slot], uy = vy[slot], uz = vz[slot];
-        uint32_t hits;
-        const uint32_t k = move_walls(I, x, y, z, ux, uy, uz, I.dt, slot, stepno, hits, acc);
-        px[slot] = x;
-        py[slot] = y;
-        pz[slot] = z;
-        vx[slot] = ux;
-        vy[slot] = uy;
-        vz[slot] = uz;
-        key[slot] = k;
-        if (k != kInvalidKey) gmem(I.rank)[slot] = gatomic_add(gmem(I.count) + k, 1u);
-        if (flags & kFlagHits) gmem(I.hits)[slot] = hits;
-      }
-      __syncthreads();
+  const gptr<const uint32_t> list = gmem<const uint32_t>(I.defer_list);
+  WallAcc acc = {0, 0, 0, 0};
+  // uniform trip count over the workgroup: the counting is wave-aggregated
+  // (neighbouring list entries mostly land in the same cell)
+  for (uint32_t b = blockIdx.x * kTileThreads; b < cnt; b += I.n_wall_wg * kTileThreads) {
+    const uint32_t i = b + tid;
+    uint32_t k = kInvalidKey, slot = 0;
+    if (i < cnt) {
+      slot = list[i];
+      float x = px[slot], y = py[slot], z = pz[slot];
+      float ux = vx[slot], uy = vy[slot], uz = vz[slot];
+      uint32_t hits;
+      k = move_walls(I, x, y, z, ux, uy, uz, I.dt, slot, stepno, hits, acc);
+      px[slot] = x;
+      py[slot] = y;
+      pz[slot] = z;
+      vx[slot] = ux;
+      vy[slot] = uy;
+      vz[slot] = uz;
+      key[slot] = k;
+      if (flags & kFlagHits) gmem(I.hits)[slot] = hits;
     }
-    double v[4] = {acc.fx, acc.fy, acc.fz, acc.heat};
+    const uint32_t rk = count_rank(gmem(I.count), k);
+    if (k != kInvalidKey) gmem(I.rank)[slot] = rk;
+  }
+  long long v[4] = {acc.fx, acc.fy, acc.fz, acc.heat};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      double x = v[k];
-      for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-      if (lane == 0) s_red[wid][k] = x;
-    }
-    __syncthreads();
+  for (int k = 0; k < 4; ++k) {
+    long long x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+    if (lane == 0) s_red[wid][k] = x;
+  }
+  __syncthreads();
+  if (tid < 4) {
+    long long t = 0;
+    for (uint32_t q = 0; q < kTileThreads / 64; ++q) t += s_red[q][tid];
+    if (t) __hip_atomic_fetch_add(gmem(I.wall_fix) + tid, (unsigned long long)t, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    __threadfence();
+    const unsigned long long done = __hip_atomic_fetch_add(gmem(I.wall_fix) + 4, 1ull, __ATOMIC_ACQ_REL,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+    s_last = done + 1ull == (unsigned long long)I.n_wall_wg ? 1u : 0u;
+  }
+  __syncthreads();
+  if (s_last) {
+    // every workgroup's sums are in: convert this step's and start the next
     if (tid < 4) {
-      double t = 0.0;
-      for (uint32_t w = 0; w < kTileThreads / 64; ++w) t += s_red[w][tid];
-      gmem(I.tile_acc)[(size_t)tile * 4 + tid] += t;
+      const long long t = (long long)__hip_atomic_exchange(gmem(I.wall_fix) + tid, 0ull, __ATOMIC_ACQ_REL,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+      gmem(I.wall_acc)[tid] += (double)t * I.wall_unit[tid < 3 ? 0 : 1];
     }
-    __syncthreads();
+    if (tid == 0) __hip_atomic_store(gmem(I.wall_fix) + 4, 0ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -1283,6 +1281,7 @@ void fix_segments(const uint32_t* start, uint32_t n_cells, uint32_t* perm, hipSt
 }
 
 void walls(const InstDesc* d, uint32_t opar, uint32_t flags, uint32_t grid, hipStream_t st, uint32_t n_inst) {
+  hipLaunchKernelGGL(k_defer_list, dim3(grid, n_inst), dim3(kTileThreads), 0, st, d);
   hipLaunchKernelGGL(k_walls, dim3(grid, n_inst), dim3(kTileThreads), 0, st, d, opar, flags);
 }
 

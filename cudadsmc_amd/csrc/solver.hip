@@ -571,6 +571,10 @@ static std::vector<uint32_t> tile_locality_order(uint32_t X, uint32_t Y, uint32_
   return order;
 }
 
+namespace {
+uint32_t wall_grid(const InstDesc& I);
+}  // namespace
+
 static int finish_loading(dsmc_solver* h, uint32_t n, uint32_t* n_out) {
   InstDesc& I = h->h;
   hipStream_t st = h->stream;
@@ -584,13 +588,28 @@ static int finish_loading(dsmc_solver* h, uint32_t n, uint32_t* n_out) {
     I.tile_order = d;
     CHECK_RC(h->upload_desc());
   }
-  if (h->physics() && I.has_geom && !I.near_tiles) {
-    // wall pass queue and per-tile wall force sums (tile count is fixed now)
-    CHECK_RC(h->alloc(&I.near_tiles, I.ntiles));
-    CHECK_RC(h->alloc(&I.defer_bits, (size_t)I.cap / 64 + 2));
-    HIPC(hipMemsetAsync(I.defer_bits, 0, ((size_t)I.cap / 64 + 2) * 8, st));
-    CHECK_RC(h->alloc(&I.tile_acc, (size_t)I.ntiles * 4));
-    HIPC(hipMemsetAsync(I.tile_acc, 0, (size_t)I.ntiles * 4 * sizeof(double), st));
+  if (h->physics() && I.has_geom && !I.defer_bits) {
+    // deferred-slot bitmask and the wall pass's force sums; the pass's grid
+    // is fixed here so that its per-workgroup sums are deterministic
+    const size_t words = (size_t)I.cap / 64 + 2;
+    CHECK_RC(h->alloc(&I.defer_bits, words));
+    HIPC(hipMemsetAsync(I.defer_bits, 0, words * 8, st));
+    CHECK_RC(h->alloc(&I.defer_list, I.cap));
+    I.n_wall_wg = wall_grid(I);
+    CHECK_RC(h->alloc(&I.wall_fix, 8));
+    HIPC(hipMemsetAsync(I.wall_fix, 0, 8 * 8, st));
+    CHECK_RC(h->alloc(&I.wall_acc, 4));
+    HIPC(hipMemsetAsync(I.wall_acc, 0, 4 * sizeof(double), st));
+    // fixed-point units of the wall sums, from the largest of the speeds
+    // that set the scale of the flow
+    const double umag = std::sqrt((double)I.U[0] * I.U[0] + (double)I.U[1] * I.U[1] + (double)I.U[2] * I.U[2]);
+    const double vref = std::max({umag, (double)I.vm_inf, (double)I.vm_wall, 1.0});
+    // (a term is at most ~2^40 (momentum) or 2^43 (energy) for speeds up to
+    // 20 v_ref, so 2^20 wall hits per step stay below 2^63)
+    I.wall_scale[0] = std::ldexp(1.0, 36) / vref;
+    I.wall_scale[1] = std::ldexp(1.0, 33) / (vref * vref);
+    I.wall_unit[0] = (double)I.mass * vref / std::ldexp(1.0, 36);
+    I.wall_unit[1] = (double)I.mass * vref * vref / std::ldexp(1.0, 34);
     CHECK_RC(h->upload_desc());
   }
   uint32_t ctr[2] = {n, n};
@@ -827,20 +846,16 @@ static int sum_acc(dsmc_solver* h, double out[kAccSlots]) {
   InstDesc& I = h->h;
   std::vector<double> acc((size_t)I.n_vwg * kAccSlots);
   HIPC(hipMemcpyAsync(acc.data(), I.wg_acc, acc.size() * 8, hipMemcpyDeviceToHost, h->stream));
-  std::vector<double> tacc(I.tile_acc ? (size_t)I.ntiles * 4 : 0);
-  if (I.tile_acc) HIPC(hipMemcpyAsync(tacc.data(), I.tile_acc, tacc.size() * 8, hipMemcpyDeviceToHost, h->stream));
+  double tacc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (I.wall_acc) HIPC(hipMemcpyAsync(tacc, I.wall_acc, sizeof(tacc), hipMemcpyDeviceToHost, h->stream));
   HIPC(hipStreamSynchronize(h->stream));
   for (int k = 0; k < kAccSlots; ++k) {
     double t = 0.0;
     for (uint32_t w = 0; w < I.n_vwg; ++w) t += acc[(size_t)w * kAccSlots + k];
     out[k] = t;
   }
-  // wall forces and heat of the wall pass, summed over tiles in order
-  for (int k = 0; k < 4 && I.tile_acc; ++k) {
-    double t = 0.0;
-    for (uint32_t tl = 0; tl < I.ntiles; ++tl) t += tacc[(size_t)tl * 4 + k];
-    out[kAccFx + k] += t;
-  }
+  // wall forces and heat of the wall pass
+  for (int k = 0; k < 4; ++k) out[kAccFx + k] += tacc[k];
   return DSMC_OK;
 }
 
@@ -1330,11 +1345,12 @@ uint32_t step_flags(const dsmc_solver* h, uint64_t stepno, bool* sample) {
 
 // Enqueue the kernels of one step reading buffer `par` (no host sync, no
 // memsets: safe to capture in a HIP graph).
-// Workgroups of the wall pass (each walks the queued tiles of its instance
-// with a stride of the grid): an eighth of the tiles, within [64, 4096] and
-// at most the tile count.
+// Workgroups of the wall pass (both launches): one wave per chunk of 64
+// bitmask words for small instances, at most 1024 workgroups (about what
+// the GPU keeps resident at the pass's occupancy); grid-stride beyond.
 uint32_t wall_grid(const InstDesc& I) {
-  return std::max(1u, std::min(I.ntiles, std::min(4096u, std::max(64u, I.ntiles / 8u))));
+  const uint64_t chunks = ((uint64_t)I.cap / 64 + 2 + 63) / 64;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(1024, (chunks + 3) / 4));
 }
 
 int enqueue_step(dsmc_solver* h, uint32_t par, uint32_t flags, hipEvent_t* ev) {
@@ -1347,7 +1363,7 @@ int enqueue_step(dsmc_solver* h, uint32_t par, uint32_t flags, hipEvent_t* ev) {
   if (ev) HIPC(hipEventRecord(ev[2], st));
   dsmc_launch::tile(I.mode, h->d_desc, par, flags, I.n_vwg, st);
   if (ev) HIPC(hipEventRecord(ev[3], st));
-  if (I.near_tiles) dsmc_launch::walls(h->d_desc, par ^ 1u, flags, wall_grid(I), st);
+  if (I.defer_bits) dsmc_launch::walls(h->d_desc, par ^ 1u, flags, I.n_wall_wg, st);
   if (ev) HIPC(hipEventRecord(ev[4], st));
   if (I.n_face_cells) {
     dsmc_launch::inject_count(h->d_desc, std::min(I.n_streams, I.n_face_cells), st);
@@ -1485,7 +1501,7 @@ int dsmc_batch_create(dsmc_solver* const* hs, uint32_t n, dsmc_batch** out) {
     b->max_vwg = std::max(b->max_vwg, h->h.n_vwg);
     b->max_inj_threads = std::max(b->max_inj_threads, std::min(h->h.n_streams, h->h.n_face_cells));
     b->scatter_grid = std::max(b->scatter_grid, h->scatter_grid);
-    if (h->h.near_tiles) b->wall_grid = std::max(b->wall_grid, wall_grid(h->h));
+    if (h->h.defer_bits) b->wall_grid = std::max(b->wall_grid, h->h.n_wall_wg);
   }
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
