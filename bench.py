@@ -193,6 +193,8 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized buffer 5 times (calibrates HBM byte counters)")
     ap.add_argument("--ensemble-steps", type=int, default=300)
+    ap.add_argument("--scatter-grid", type=int, default=None,
+                    help="workgroups of the scatter kernel (default: tuned, else the engine's)")
     ap.add_argument("--cg-refine", type=int, default=None,
                     help="collision-grid cells per flow cell along each axis (default: tuned, else 1)")
     args = ap.parse_args()
@@ -227,8 +229,9 @@ def main() -> int:
     t_setup = time.perf_counter()
     sol = dm.Solver()
     sol.Init(s)
-    if ens.tuned(args.workload, 0, "scatter_grid"):
-        sol.set_scatter_grid(int(ens.tuned(args.workload, 0, "scatter_grid")))
+    sg = args.scatter_grid or int(ens.tuned(args.workload, 0, "scatter_grid"))
+    if sg:
+        sol.set_scatter_grid(sg)
     sol.SetModel(mesh, grid=cgrid)
     n0 = sol.InitSystemState(tuple(s.stream_velocity))
     sol.synchronize()
@@ -302,7 +305,7 @@ def main() -> int:
     sol.close()
     if not args.no_ensemble:
         t0 = time.perf_counter()
-        line["ensemble"] = ensemble_bench(args.ensemble_steps, 100)
+        line["ensemble"] = ensemble_bench(args.ensemble_steps, 500)
         print(f"[bench] ensemble {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()

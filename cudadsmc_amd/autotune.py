@@ -124,7 +124,7 @@ def main(argv=None) -> None:
         if what == "ensemble64":
             cases = ens.draw_cases(64, seed=2024, temperatures=False)
             res = tune_batch(lambda ns: [ens.make_solver(c, n_streams=ns) for c in cases],
-                             streams=(8192, 16384, 32768, 65536), warmup=30, steps=60)
+                             streams=(8192, 16384, 32768, 65536), warmup=300, steps=60)
         elif what == "c3":
             import sys
 
@@ -144,16 +144,17 @@ def main(argv=None) -> None:
 
             # collision-grid refinement first (engine-default streams), then
             # the launch layout on the chosen grid
+            # timed on the developed flow (shock layer and wake formed)
             rtab = []
             for refine in (1, 2, 3, 4):
                 sol = make(0, refine)
                 dt = sol.settings.dt
-                t = _time_steps(lambda k: sol.DoBenchmark(k, dt), sol.synchronize, 20, 40)
+                t = _time_steps(lambda k: sol.DoBenchmark(k, dt), sol.synchronize, 600, 40)
                 sol.close()
                 rtab.append({"cg_refine": refine, "ms_per_step": 1e3 * t})
             refine = min(rtab, key=lambda r: r["ms_per_step"])["cg_refine"]
             res = tune_solver(lambda ns: make(ns, refine), streams=(262144, 524288, 1048576, 2097152),
-                              scatter=(1024, 2048, 4096, 8192), warmup=20, steps=40)
+                              scatter=(1024, 2048, 4096, 8192), warmup=600, steps=40)
             res["cg_refine"] = refine
             res["table"] = rtab + res["table"]
         else:
