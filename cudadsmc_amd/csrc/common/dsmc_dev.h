@@ -118,7 +118,9 @@ struct InstDesc {
   float4* cstat;        // REFERENCE per-cell {selections, collisions, sum c_r, 0}
   double* samp;         // [5][ncells]
   float* fluid_frac;    // PHYSICS per-cell fluid volume fraction
-  const uint8_t* near_cell;  // PHYSICS with a body: cells within one cell of a collision-grid cell in use
+  const uint8_t* near_cell;  // PHYSICS with a body: near-body code per cell (255: far; else the distance
+                             // to the body's collision-grid cells in units of near_unit)
+  float near_unit;      // smallest cell size / 64
   // ---- scalars and per-workgroup accumulators
   uint32_t* ctr;        // [0] stored, [1] sorted, [2] injected particles, [3] status bits, [4] step number,
                         // [5] deferred particles of the step

@@ -457,10 +457,17 @@ bool point_inside(const Mesh& m, const CGrid& g, float x, float y, float z) {
   return sd < 0.0f;
 }
 
+// Near-body code of every flow cell: 255 when no occupied collision-grid
+// cell lies within the cell's box widened by one cell per axis (a particle
+// that moves less than a cell along every axis cannot reach the body), else
+// the Euclidean distance from the cell's box to the nearest occupied
+// collision-grid cell, in units of (smallest cell size) / 64, rounded down
+// and capped at 254 (so the code never overstates the distance).
 void near_cells(const CGrid& g, const float gmin[3], const float cell[3], const uint32_t dim[3],
                 std::vector<uint8_t>& out) {
   const uint64_t C = (uint64_t)dim[0] * dim[1] * dim[2];
-  out.assign(C, 0);
+  out.assign(C, 255);
+  const float unit = std::min(cell[0], std::min(cell[1], cell[2])) / 64.0f;
   // collision-grid index range covered by sim cells [i - 1, i + 2) per axis
   auto range = [&](int k, uint32_t i, int& lo, int& hi) {
     double a = (double)gmin[k] + ((double)i - 1.0) * cell[k];
@@ -470,6 +477,8 @@ void near_cells(const CGrid& g, const float gmin[3], const float cell[3], const 
     lo = std::max(lo, 0);
     hi = std::min(hi, (int)g.dim[k] - 1);
   };
+  // gap between [a0, a1] and [b0, b1] along one axis (0 when they overlap)
+  auto gap = [](double a0, double a1, double b0, double b1) { return std::max(0.0, std::max(b0 - a1, a0 - b1)); };
   std::vector<uint8_t> occ(g.ncells());
   for (uint32_t c = 0; c < g.ncells(); ++c) occ[c] = g.flag[c] != 0u;
   for (uint32_t z = 0; z < dim[2]; ++z) {
@@ -484,12 +493,28 @@ void near_cells(const CGrid& g, const float gmin[3], const float cell[3], const 
         int x0, x1;
         range(0, x, x0, x1);
         if (x0 > x1) continue;
-        bool hit = false;
-        for (int cz = z0; cz <= z1 && !hit; ++cz)
-          for (int cy = y0; cy <= y1 && !hit; ++cy)
-            for (int cx = x0; cx <= x1 && !hit; ++cx)
-              hit = occ[(uint64_t)cz * g.dim[0] * g.dim[1] + (uint64_t)cy * g.dim[0] + cx] != 0;
-        out[(uint64_t)z * dim[0] * dim[1] + (uint64_t)y * dim[0] + x] = hit ? 1 : 0;
+        const double bx0 = (double)gmin[0] + (double)x * cell[0], bx1 = bx0 + cell[0];
+        const double by0 = (double)gmin[1] + (double)y * cell[1], by1 = by0 + cell[1];
+        const double bz0 = (double)gmin[2] + (double)z * cell[2], bz1 = bz0 + cell[2];
+        double best = -1.0;
+        for (int cz = z0; cz <= z1; ++cz) {
+          const double qz0 = (double)g.mn[2] + (double)cz * g.cell[2];
+          const double gz = gap(bz0, bz1, qz0, qz0 + g.cell[2]);
+          for (int cy = y0; cy <= y1; ++cy) {
+            const double qy0 = (double)g.mn[1] + (double)cy * g.cell[1];
+            const double gy = gap(by0, by1, qy0, qy0 + g.cell[1]);
+            for (int cx = x0; cx <= x1; ++cx) {
+              if (!occ[(uint64_t)cz * g.dim[0] * g.dim[1] + (uint64_t)cy * g.dim[0] + cx]) continue;
+              const double qx0 = (double)g.mn[0] + (double)cx * g.cell[0];
+              const double gx = gap(bx0, bx1, qx0, qx0 + g.cell[0]);
+              const double d = std::sqrt(gx * gx + gy * gy + gz * gz);
+              if (best < 0.0 || d < best) best = d;
+            }
+          }
+        }
+        if (best >= 0.0)
+          out[(uint64_t)z * dim[0] * dim[1] + (uint64_t)y * dim[0] + x] =
+              (uint8_t)std::min(254.0, std::floor(best / (double)unit));
       }
     }
   }

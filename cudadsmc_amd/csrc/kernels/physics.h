@@ -391,17 +391,23 @@ struct WallAcc {
   long long fx, fy, fz, heat;
 };
 
-// PHYSICS move. A particle is "near" when its cell is flagged as within one
-// cell of the body's collision-grid cells, or when it moves a cell or more
-// along some axis in one step: its wall interaction is deferred to the wall
-// pass (k_walls), which runs after the tile kernel with the particle's state
-// as left there. Every other particle can reach no triangle and only flies;
-// for a near particle that hits nothing the wall pass computes exactly the
-// same flight, so the split changes no result.
-__device__ __forceinline__ bool phys_near(const InstDesc& I, bool near_cell, float vx, float vy, float vz,
+// PHYSICS move. A particle is "near" when it moves a cell or more along some
+// axis in one step, or when its cell has a near-body code (the distance from
+// the cell to the body's collision-grid cells, in units of near_unit) and
+// its flight |v| dt is at least that distance. Its wall interaction is then
+// deferred to the wall pass (k_walls), which runs after the tile kernel
+// with the particle's state as left there. Every other particle can reach
+// no triangle and only flies; for a near particle that hits nothing the
+// wall pass computes exactly the same flight, so the split changes no
+// result.
+__device__ __forceinline__ bool phys_near(const InstDesc& I, uint32_t code, float vx, float vy, float vz,
                                           float dt) {
   if (!I.has_geom) return false;
-  return near_cell || fabsf(vx) * dt >= I.cell[0] || fabsf(vy) * dt >= I.cell[1] || fabsf(vz) * dt >= I.cell[2];
+  if (fabsf(vx) * dt >= I.cell[0] || fabsf(vy) * dt >= I.cell[1] || fabsf(vz) * dt >= I.cell[2]) return true;
+  if (code == 255u) return false;
+  const float s2 = (vx * vx + vy * vy + vz * vz) * (dt * dt);
+  const float d = (float)code * I.near_unit;
+  return s2 >= d * d;
 }
 
 // Box faces after the flight: the new cell key, or kInvalidKey when the

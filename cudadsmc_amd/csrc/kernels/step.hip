@@ -531,13 +531,13 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
   // the next tile's cell offsets and near flags, stored to the other LDS
   // buffer at the end of this phase: the next tile starts without a load
   uint32_t nx_start = 0u, nx_end = 0u, ncn = 0u;
-  uint8_t nx_near = 0u;
+  uint8_t nx_near = 255u;
   if (tile_n != kInvalidKey) {
     const uint32_t c0n = tile_n * I.tile_cells;
     ncn = min(I.tile_cells, I.ncells - c0n);
     nx_start = gmem(I.cell_start)[c0n + min(tid, ncn)];
     if (tid == 0) nx_end = gmem(I.cell_start)[c0n + ncn];
-    if (MODE == kModePhysics) nx_near = (I.near_cell && tid < ncn) ? gmem(I.near_cell)[c0n + tid] : (uint8_t)0;
+    if (MODE == kModePhysics) nx_near = (I.near_cell && tid < ncn) ? gmem(I.near_cell)[c0n + tid] : (uint8_t)255;
   }
   // (4) NTC collisions, one lane per cell
   if (tid < nc) {
@@ -616,7 +616,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
       if (MODE == kModeReference) {
         hits = move_reference(I, x, y, z, ux, uy, uz, I.dt);
         key = cell_key(I, x, y, z);
-      } else if (phys_near(I, s_nearc[own[q]] != 0, ux, uy, uz, I.dt)) {
+      } else if (phys_near(I, s_nearc[own[q]], ux, uy, uz, I.dt)) {
         key = kDeferredKey;  // stored as is; k_walls moves it and counts its key
       } else {
         key = move_free(I, x, y, z, ux, uy, uz, I.dt);
@@ -735,7 +735,7 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
     s_start[0][tid] = gmem(I.cell_start)[c0 + min(tid, nc)];
     if (tid == 0) s_start[0][nc] = gmem(I.cell_start)[c0 + nc];
     if (MODE == kModePhysics)
-      s_nearc[0][tid] = (I.near_cell && tid < nc) ? gmem(I.near_cell)[c0 + tid] : (uint8_t)0;
+      s_nearc[0][tid] = (I.near_cell && tid < nc) ? gmem(I.near_cell)[c0 + tid] : (uint8_t)255;
   }
   __syncthreads();
   uint32_t pf[kPermPrefetch];

@@ -535,6 +535,7 @@ int dsmc_set_geometry(dsmc_solver* h, const float* verts, uint32_t nv, const uin
     HIPC(hipMemcpyAsync(df, h->frac.data(), (size_t)I.ncells * 4, hipMemcpyHostToDevice, st));
     I.fluid_frac = df;
     near_cells(g, I.gmin, I.cell, I.dim, h->nearc);
+    I.near_unit = std::min(I.cell[0], std::min(I.cell[1], I.cell[2])) / 64.0f;
     uint8_t* dn;
     CHECK_RC(h->alloc(&dn, I.ncells, true));
     HIPC(hipMemcpyAsync(dn, h->nearc.data(), (size_t)I.ncells, hipMemcpyHostToDevice, st));

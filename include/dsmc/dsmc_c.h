@@ -211,8 +211,12 @@ int dsmc_set_tile_cells(dsmc_solver* h, uint32_t tile_cells);
  * 912-1000, seeded per cell as CreateRandomSeedsUint4 does at :1846-1886). */
 int dsmc_get_taus_state(dsmc_solver* h, uint32_t* out, uint32_t n_words);
 int dsmc_set_taus_state(dsmc_solver* h, const uint32_t* in, uint32_t n_words);
-/* PHYSICS with a body: 1 for the cells within one cell of the body's
- * collision-grid cells (their particles go through the wall pass), else 0. */
+/* PHYSICS with a body: near-body code per cell. 255 when no collision-grid
+ * cell holding part of the body lies within the cell's box widened by one
+ * cell per axis; else the distance from the cell's box to the nearest such
+ * collision-grid cell in units of (smallest cell size) / 64, rounded down
+ * (at most 254). A particle goes through the wall pass when its flight
+ * |v| dt reaches that distance or it moves a cell or more along an axis. */
 int dsmc_get_near_cells(dsmc_solver* h, uint8_t* out, uint32_t n_cells);
 /* Per-cell fluid volume fraction used by PHYSICS mode (1 = no body). */
 int dsmc_get_fluid_fraction(dsmc_solver* h, float* out, uint32_t n_cells);

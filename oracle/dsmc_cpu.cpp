@@ -182,7 +182,7 @@ struct oracle_sim {
   uint32_t cgdim[3];
   std::vector<uint32_t> cgflag, cgstart, cgend, cgtri;
   std::vector<float> frac;
-  std::vector<uint8_t> nearc;  // PHYSICS: cells whose particles go through the wall pass
+  std::vector<uint8_t> nearc;  // PHYSICS: near-body code per cell (the engine's, see near_body)
   // RNG streams
   std::vector<MtBlock> rng;
   std::vector<TausState> taus;  // per cell, REFERENCE mode with mt = 0 (empty otherwise)
@@ -402,13 +402,20 @@ struct oracle_sim {
     return true;
   }
 
-  // PHYSICS move, in two parts as the engine runs it: a particle whose
-  // free-flight segment box meets the collision grid's box is deferred to
-  // the wall pass (move_walls, after all tiles); all others just fly.
+  // PHYSICS move, in two parts as the engine runs it: a particle that moves
+  // a cell or more along some axis, or whose flight |u| dt reaches its
+  // cell's near-body distance (code * smallest cell / 64; 255 = far), is
+  // deferred to the wall pass (move_walls, after all tiles); all others
+  // just fly.
   bool near_body(uint32_t c, float ux, float uy, float uz) const {
     if (!has_geom) return false;
-    bool flagged = nearc.empty() || nearc[c] != 0;  // without the engine's flags: every cell
-    return flagged || std::fabs(ux) * dt >= cell[0] || std::fabs(uy) * dt >= cell[1] || std::fabs(uz) * dt >= cell[2];
+    if (std::fabs(ux) * dt >= cell[0] || std::fabs(uy) * dt >= cell[1] || std::fabs(uz) * dt >= cell[2]) return true;
+    const uint32_t code = nearc.empty() ? 0u : nearc[c];  // without the engine's codes: every cell at distance 0
+    if (code == 255u) return false;
+    const float unit = std::min(cell[0], std::min(cell[1], cell[2])) / 64.0f;
+    const float s2 = (ux * ux + uy * uy + uz * uz) * (dt * dt);
+    const float d = (float)code * unit;
+    return s2 >= d * d;
   }
 
   uint32_t faces_key(float& px, float& py, float& pz, float& ux, float& uy, float& uz) const {

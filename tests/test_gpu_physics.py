@@ -179,3 +179,49 @@ def test_deterministic_replay():
         sol.close()
     for k in ("pos", "vel", "key"):
         assert np.array_equal(runs[0][k], runs[1][k])
+
+
+def test_near_body_codes():
+    """Near-body codes against a brute-force distance from every cell box to
+    the occupied collision-grid cells: a code never overstates the distance
+    (a particle that could reach the body is always deferred), 255 only
+    where nothing occupied lies within one cell, and the codes are the
+    rounded-down distances."""
+    note("near-body codes")
+    s = small_physics_settings(dims=(24, 16, 16), n_target=20000, omega=0.81, wall_temp=300.0, U=1500.0,
+                               density=1.0e20)
+    sol = dm.Solver()
+    sol.Init(s)
+    mesh = dm.icosphere((0.06, 0.05, 0.05), 0.018, 2)
+    sol.SetModel(mesh, grid=(13, 13, 13))
+    code = sol.near_cells().astype(np.int64)
+    g = sol.collision_grid()
+    b, dims = g["bounds"].astype(np.float64), g["dims"].astype(np.int64)
+    gmin, gcell = b[0:3], b[6:9]
+    X, Y, Z = (int(d) for d in s.grid_dim)
+    cell = np.array([s.cell_size[0], s.cell_size[1], s.cell_size[2]], np.float64)
+    unit = float(np.float32(np.float32(min(s.cell_size)) / np.float32(64.0)))
+    occ = np.nonzero(g["flags"] != 0)[0]
+    ox, oy, oz = occ % dims[0], (occ // dims[0]) % dims[1], occ // (dims[0] * dims[1])
+    q0 = np.stack([gmin[0] + ox * gcell[0], gmin[1] + oy * gcell[1], gmin[2] + oz * gcell[2]], 1)
+    q1 = q0 + gcell
+    cz, cy, cx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    b0 = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], 1) * cell + np.array(s.bmin[:3], np.float64)
+    b1 = b0 + cell
+    gap = np.maximum(0.0, np.maximum(q0[None] - b1[:, None], b0[:, None] - q1[None]))  # (cells, occ, 3)
+    d = np.sqrt((gap ** 2).sum(-1))
+    within = (gap <= cell * (1 + 1e-6)).all(-1)  # inside the cell's box widened by one cell
+    d_all = d.min(1)
+    d_win = np.where(within, d, np.inf).min(1)
+    near = code != 255
+    assert near.sum() > 0
+    # "far" only where nothing occupied lies within the widened box (a particle
+    # would have to move a cell or more along some axis, which always defers it)
+    assert np.all(~np.isfinite(d_win[~near]))
+    assert np.all(np.isfinite(d_win[near]) | (d_all[near] > 0))
+    # never more than the distance to an occupied cell of the widened box, so a
+    # particle whose flight reaches one is deferred; never less than the
+    # rounded-down distance to the nearest occupied cell overall
+    fin = near & np.isfinite(d_win)
+    assert np.all(code[fin] * unit <= d_win[fin] * (1 + 1e-6) + 1e-12)
+    assert np.all(code[near] >= np.minimum(254, np.floor(d_all[near] / unit * (1 - 1e-6))))
