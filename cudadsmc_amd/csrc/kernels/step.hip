@@ -630,9 +630,11 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
         ovy[q] = uy;
         ovz[q] = uz;
       }
-      vx[q] = ux;
-      vy[q] = uy;
-      vz[q] = uz;
+      if (!LDS || (flags & kFlagSample)) {  // post-move velocities: output (global path) or sampled
+        vx[q] = ux;
+        vy[q] = uy;
+        vz[q] = uz;
+      }
       if (flags & kFlagHits) gmem(I.hits)[p0 + q] = hits;
       if (flags & kFlagKeepPerm) gmem(I.perm)[p0 + q] = src;
     }
@@ -829,7 +831,7 @@ __global__ void __launch_bounds__(kTileThreads) k_defer_list(const InstDesc* __r
 // order in which particles are listed and run does not change the sums;
 // the last workgroup to finish converts the step's sums and adds them to
 // wall_acc.
-__global__ void __launch_bounds__(kTileThreads) k_walls(const InstDesc* __restrict__ Ip, uint32_t opar,
+__global__ void __launch_bounds__(kTileThreads, 4) k_walls(const InstDesc* __restrict__ Ip, uint32_t opar,
                                                        uint32_t flags) {
   const InstDesc& I = Ip[blockIdx.y];
   if (!I.defer_bits || blockIdx.x >= I.n_wall_wg) return;
