@@ -1104,6 +1104,11 @@ __global__ void k_inject_generate(const InstDesc* __restrict__ Ip, uint32_t opar
     I.ctr[2] += end - base;
   }
   if (s >= I.n_streams || s >= I.n_face_cells) return;
+  // most face cells inject nothing in a step (outflow and side faces): such
+  // threads draw nothing and leave their stream untouched
+  bool any = false;
+  for (uint32_t g = s; g < I.n_face_cells && !any; g += I.n_streams) any = I.face_count[g + 1] > I.face_count[g];
+  if (!any) return;
   Mt rng;
   mt_load(I, s, rng);
   const float twopi = 6.28318530717958647692f;
