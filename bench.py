@@ -139,7 +139,7 @@ def ensemble_bench(steps: int, warmup: int, instances: int = 64) -> dict:
     particle counts log-uniform in [1e4, 5e5], U in [600, 3000] m/s, n in
     [3e19, 1e21] m^-3, r in [0.015, 0.035] m, seeds 0..instances-1, advanced
     by the batched step with HIP graph replay; RNG streams per instance from
-    the autotuner's table."""
+    the autotuner's table, and cells per tile likewise."""
     import cudadsmc_amd as dm
     from cudadsmc_amd import ensemble as ens
 
@@ -147,6 +147,7 @@ def ensemble_bench(steps: int, warmup: int, instances: int = 64) -> dict:
     cases = ens.draw_cases(instances, seed=2024, temperatures=False)
     t0 = time.perf_counter()
     sols = [ens.make_solver(c, n_streams=ns) for c in cases]
+    tcs = [s.layout()["tile_cells"] for s in sols]
     b = dm.Batch(sols)
     b.synchronize()
     setup = time.perf_counter() - t0
@@ -167,7 +168,7 @@ def ensemble_bench(steps: int, warmup: int, instances: int = 64) -> dict:
     return {"instances": instances, "particles_total": navg, "cells_total": cells, "steps": steps,
             "warmup": warmup, "ms_per_step": 1e3 * el / steps, "particle_steps_per_s": ps,
             "pct_hbm_peak": 100.0 * (B_P * navg + B_C * cells) * steps / el / HBM_PEAK,
-            "n_streams_per_instance": ns, "setup_s": setup,
+            "n_streams_per_instance": ns, "tile_cells": sorted({int(t) for t in tcs}), "setup_s": setup,
             "N_range": [min(c.N for c in cases), max(c.N for c in cases)]}
 
 
@@ -197,6 +198,8 @@ def main() -> int:
                     help="workgroups of the scatter kernel (default: tuned, else the engine's)")
     ap.add_argument("--cg-refine", type=int, default=None,
                     help="collision-grid cells per flow cell along each axis (default: tuned, else 1)")
+    ap.add_argument("--tile-cells", type=int, default=None,
+                    help="cells per tile of the tile kernel (default: tuned, else the engine's sizing)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition is "
@@ -229,6 +232,9 @@ def main() -> int:
     t_setup = time.perf_counter()
     sol = dm.Solver()
     sol.Init(s)
+    tc = args.tile_cells or int(ens.tuned(args.workload, 0, "tile_cells"))
+    if tc:
+        sol.set_tile_cells(tc)
     sg = args.scatter_grid or int(ens.tuned(args.workload, 0, "scatter_grid"))
     if sg:
         sol.set_scatter_grid(sg)
@@ -290,6 +296,7 @@ def main() -> int:
             "particles_avg": n_avg,
             "cells": cells,
             "collision_grid": list(cgrid),
+            "tile_cells": int(sol.layout()["tile_cells"]),
             "parallelism": "single GPU",
         },
         "pct_hbm_peak": 100.0 * hbm_step / HBM_PEAK,

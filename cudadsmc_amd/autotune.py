@@ -1,6 +1,6 @@
 """Autotuner of the DSMC step's launch layout.
 
-Two parameters are tuned by timing the real case on the GPU:
+Launch parameters are tuned by timing the real case on the GPU:
 
 * ``n_streams``: the number of RNG streams, which fixes the tile kernel's
   grid (n_streams / 256 workgroups, each walking tiles vwg, vwg + G, ...).
@@ -10,6 +10,12 @@ Two parameters are tuned by timing the real case on the GPU:
   is reproducible for a given value).
 * ``scatter_grid``: workgroups of the grid-stride scatter kernel (does not
   change results).
+
+For the ensemble it also picks the cells per tile: its instances are small
+(65536 cells), so the engine's sizing (tiles of about 1000 expected
+particles) leaves about one tile per workgroup and no next tile to prefetch;
+smaller tiles give each workgroup a walk of several tiles. Like n_streams,
+the tile size fixes the stream assignment.
 
 For the large single case it also picks the collision-grid refinement (cells
 of the triangle grid per flow cell along each axis): a finer grid holds
@@ -77,24 +83,28 @@ def tune_solver(make: Callable[[int], "object"], streams: Iterable[int] = DEFAUL
     return out
 
 
-def tune_batch(make_members: Callable[[int], list], streams: Iterable[int] = (4096, 8192, 16384, 32768, 65536),
-               warmup: int = 10, steps: int = 20) -> dict:
-    """Best per-instance n_streams for an ensemble stepped as one batch."""
+def tune_batch(make_members: Callable[[int, int], list], streams: Iterable[int] = (4096, 8192, 16384, 32768, 65536),
+               tiles: Iterable[int] = (0,), warmup: int = 10, steps: int = 20) -> dict:
+    """Best per-instance (n_streams, tile_cells) for an ensemble stepped as
+    one batch; ``make_members(n_streams, tile_cells)`` builds the loaded
+    members (tile_cells 0: the engine's sizing)."""
     import cudadsmc_amd as dm
 
     table = []
     best = None
-    for ns in streams:
-        members = make_members(int(ns))
-        b = dm.Batch(members)
-        t = _time_steps(b.run, b.synchronize, warmup, steps)
-        table.append({"n_streams": int(ns), "ms_per_step": 1e3 * t})
-        if best is None or t < best[0]:
-            best = (t, int(ns))
-        b.close()
-        for m in members:
-            m.close()
-    return {"n_streams": best[1], "ms_per_step": 1e3 * best[0], "table": table}
+    for tc in tiles:
+        for ns in streams:
+            members = make_members(int(ns), int(tc))
+            b = dm.Batch(members)
+            t = _time_steps(b.run, b.synchronize, warmup, steps)
+            table.append({"n_streams": int(ns), "tile_cells": int(tc), "ms_per_step": 1e3 * t})
+            print("ensemble", json.dumps(table[-1]), flush=True)
+            if best is None or t < best[0]:
+                best = (t, int(ns), int(tc))
+            b.close()
+            for m in members:
+                m.close()
+    return {"n_streams": best[1], "tile_cells": best[2], "ms_per_step": 1e3 * best[0], "table": table}
 
 
 def _load(path) -> dict:
@@ -123,8 +133,8 @@ def main(argv=None) -> None:
     for what in a.what.split(","):
         if what == "ensemble64":
             cases = ens.draw_cases(64, seed=2024, temperatures=False)
-            res = tune_batch(lambda ns: [ens.make_solver(c, n_streams=ns) for c in cases],
-                             streams=(8192, 16384, 32768, 65536), warmup=300, steps=60)
+            res = tune_batch(lambda ns, tc: [ens.make_solver(c, n_streams=ns, tile_cells=tc) for c in cases],
+                             streams=(32768, 65536, 131072), tiles=(32, 64, 128, 0), warmup=300, steps=60)
         elif what == "c3":
             import sys
 

@@ -722,13 +722,13 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
   const uint32_t tid = threadIdx.x;
   const uint32_t vw = blockIdx.x;
   const uint32_t stream = vw * kTileThreads + tid;
+  if (vw == 0 && tid == 0) I.ctr[0] = I.ctr[1];
+  TileWalk w = tile_walk_begin(I.n_vwg, vw);
+  uint32_t j = tile_walk_pos(w);
+  if (j >= I.ntiles) return;  // no tile for this workgroup: its streams and sums stay as they are
   Mt rng;
   mt_load(I, stream, rng);
   CollAcc cacc = {0.0, 0.0, 0.0};
-  if (vw == 0 && tid == 0) I.ctr[0] = I.ctr[1];
-
-  TileWalk w = tile_walk_begin(I.n_vwg, vw);
-  uint32_t j = tile_walk_pos(w);
   uint32_t tile = kInvalidKey;
   if (j < I.ntiles) {
     tile = I.tile_order ? gmem(I.tile_order)[j] : j;

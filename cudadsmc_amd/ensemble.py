@@ -106,13 +106,20 @@ def collision_grid_dims(r: float) -> tuple[int, int, int]:
     return (g, g, g)
 
 
-def make_solver(case: Case, dt: float = 1.0e-6, n_streams: int | None = None, fields: dict | None = None) -> Solver:
+def make_solver(case: Case, dt: float = 1.0e-6, n_streams: int | None = None, fields: dict | None = None,
+                tile_cells: int | None = None) -> Solver:
     """A loaded solver: cold (uniform freestream) or warm (fields n, u, v,
-    w, T per cell, X fastest)."""
+    w, T per cell, X fastest). ``n_streams`` and ``tile_cells`` default to
+    the tuned ensemble layout (0 tile cells: the engine's sizing); both fix
+    the stream assignment, so results are reproducible for given values."""
     if n_streams is None:
         n_streams = tuned("ensemble64", 16384)
+    if tile_cells is None:
+        tile_cells = int(tuned("ensemble64", 0, "tile_cells"))
     sol = Solver()
     sol.Init(settings(case, dt, n_streams))
+    if tile_cells:
+        sol.set_tile_cells(tile_cells)
     sol.SetModel(icosphere(CENTER, case.r, ICO_SUBDIV), grid=collision_grid_dims(case.r))
     if fields is None:
         sol.InitSystemState((case.U, 0.0, 0.0))
