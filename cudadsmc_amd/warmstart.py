@@ -397,7 +397,7 @@ def evaluate(out: Path, n_cases: int, seed: int, N: int, ref_transient: int, ref
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("stage", choices=["data", "train", "eval", "all"])
-    ap.add_argument("--out", default="gpurun_out/warmstart")
+    ap.add_argument("--out", default="warmstart_out", help="folder for the data set, operator and results")
     ap.add_argument("--runs", type=int, default=512)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--transient", type=int, default=2000)
