@@ -337,25 +337,51 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __
   const uint32_t* ts = scratch + blockIdx.y * (dsmc_launch::kScanMaxTiles + 1);
   uint32_t* out = which ? I.face_count : I.cell_start;
   __shared__ uint32_t sh[kScanThreads / 64 + 1];
+  // each thread scans kScanItems consecutive entries; a thread whose entries
+  // all lie below n moves them as 16-byte vectors (the buffers are
+  // 256-byte aligned and base is a multiple of 16 entries)
   uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  const bool whole = base + kScanItems <= n;
   uint32_t v[kScanItems];
+  if (whole) {
+    const gptr<const uint4> src = gmem<const uint4>(reinterpret_cast<const uint4*>(in + base));
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems / 4u; ++k) {
+      const uint4 q = gload(src + k);
+      v[4 * k] = q.x;
+      v[4 * k + 1] = q.y;
+      v[4 * k + 2] = q.z;
+      v[4 * k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k) v[k] = base + k < n ? in[base + k] : 0u;
+  }
   uint32_t local = 0;
 #pragma unroll
-  for (uint32_t k = 0; k < kScanItems; ++k) {
-    uint32_t i = base + k;
-    v[k] = i < n ? in[i] : 0u;
-    local += v[k];
-  }
+  for (uint32_t k = 0; k < kScanItems; ++k) local += v[k];
   uint32_t tot;
   uint32_t off = block_exclusive_scan(local, &tot, sh) + ts[blockIdx.x];
+  if (whole) {
+    const gptr<uint4> dst = gmem(reinterpret_cast<uint4*>(out + base));
+    const gptr<uint4> clr = gmem(reinterpret_cast<uint4*>(in + base));
 #pragma unroll
-  for (uint32_t k = 0; k < kScanItems; ++k) {
-    uint32_t i = base + k;
-    if (i < n) {
-      out[i] = off;
-      if (which == 0) in[i] = 0u;
+    for (uint32_t k = 0; k < kScanItems / 4u; ++k) {
+      const uint32_t o0 = off, o1 = o0 + v[4 * k], o2 = o1 + v[4 * k + 1], o3 = o2 + v[4 * k + 2];
+      off = o3 + v[4 * k + 3];
+      gstore(dst + k, make_uint4(o0, o1, o2, o3));
+      if (which == 0) gstore(clr + k, make_uint4(0u, 0u, 0u, 0u));
     }
-    off += v[k];
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+      uint32_t i = base + k;
+      if (i < n) {
+        out[i] = off;
+        if (which == 0) in[i] = 0u;
+      }
+      off += v[k];
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = ts[dsmc_launch::kScanMaxTiles];
 }
