@@ -166,10 +166,20 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* in
   __shared__ uint32_t red[kScanThreads / 64];
   uint32_t base = blockIdx.x * kScanTile;
   uint32_t sum = 0;
+  if (base + kScanTile <= n && (reinterpret_cast<uintptr_t>(in) & 15u) == 0u) {
+    // a whole tile of a 16-byte aligned array: vector loads, consecutive lanes on consecutive vectors
+    const gptr<const uint4> src = gmem<const uint4>(reinterpret_cast<const uint4*>(in + base));
 #pragma unroll
-  for (uint32_t k = 0; k < kScanItems; ++k) {
-    uint32_t i = base + k * kScanThreads + threadIdx.x;
-    if (i < n) sum += in[i];
+    for (uint32_t k = 0; k < kScanItems / 4u; ++k) {
+      const uint4 q = gload(src + k * kScanThreads + threadIdx.x);
+      sum += (q.x + q.y) + (q.z + q.w);
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+      uint32_t i = base + k * kScanThreads + threadIdx.x;
+      if (i < n) sum += in[i];
+    }
   }
   for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off);
   if (lane_id() == 0) red[threadIdx.x / 64] = sum;
@@ -276,10 +286,20 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_reduce(const InstDesc* _
   __shared__ uint32_t red[kScanThreads / 64];
   uint32_t base = blockIdx.x * kScanTile;
   uint32_t sum = 0;
+  if (base + kScanTile <= n && (reinterpret_cast<uintptr_t>(in) & 15u) == 0u) {
+    // a whole tile of a 16-byte aligned array: vector loads, consecutive lanes on consecutive vectors
+    const gptr<const uint4> src = gmem<const uint4>(reinterpret_cast<const uint4*>(in + base));
 #pragma unroll
-  for (uint32_t k = 0; k < kScanItems; ++k) {
-    uint32_t i = base + k * kScanThreads + threadIdx.x;
-    if (i < n) sum += in[i];
+    for (uint32_t k = 0; k < kScanItems / 4u; ++k) {
+      const uint4 q = gload(src + k * kScanThreads + threadIdx.x);
+      sum += (q.x + q.y) + (q.z + q.w);
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+      uint32_t i = base + k * kScanThreads + threadIdx.x;
+      if (i < n) sum += in[i];
+    }
   }
   for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off);
   if (lane_id() == 0) red[threadIdx.x / 64] = sum;
@@ -338,10 +358,10 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __
   uint32_t* out = which ? I.face_count : I.cell_start;
   __shared__ uint32_t sh[kScanThreads / 64 + 1];
   // each thread scans kScanItems consecutive entries; a thread whose entries
-  // all lie below n moves them as 16-byte vectors (the buffers are
-  // 256-byte aligned and base is a multiple of 16 entries)
+  // all lie below n moves them as 16-byte vectors when the arrays are 16-byte
+  // aligned (the engine's are 256-byte aligned; base is a multiple of 16)
   uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  const bool whole = base + kScanItems <= n;
+  const bool whole = base + kScanItems <= n && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0u;
   uint32_t v[kScanItems];
   if (whole) {
     const gptr<const uint4> src = gmem<const uint4>(reinterpret_cast<const uint4*>(in + base));
@@ -821,8 +841,8 @@ template __global__ void k_tile<kModePhysics>(const InstDesc*, uint32_t, uint32_
 //
 // k_defer_list compacts the marked slots: each wave takes chunks of 64
 // bitmask words (4096 slots), finds their set bits with one coalesced load,
-// reserves room for them in defer_list with one atomic, writes them and
-// clears the words.
+// reserves room for them in defer_list with one atomic, writes each word's
+// slots as one contiguous run (lane l tests bit l) and clears the words.
 __global__ void __launch_bounds__(kTileThreads) k_defer_list(const InstDesc* __restrict__ Ip) {
   const InstDesc& I = Ip[blockIdx.y];
   if (!I.defer_bits || blockIdx.x >= I.n_wall_wg) return;
@@ -845,8 +865,19 @@ __global__ void __launch_bounds__(kTileThreads) k_defer_list(const InstDesc* __r
     if (total == 0u) continue;  // uniform over the wave
     uint32_t base = 0u;
     if (lane == 0) base = gatomic_add(gmem(I.ctr) + 5, total);
-    uint32_t k = __shfl(base, 0) + incl - pc;
-    for (unsigned long long m = word; m; m &= m - 1ull) list[k++] = w * 64u + (uint32_t)__ffsll(m) - 1u;
+    base = __shfl(base, 0);
+    // word r of the chunk (read from lane r): each lane tests its own bit,
+    // so the word's set slots are written as one contiguous run
+    const uint32_t ex = incl - pc;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint32_t r = 0; r < 64u; ++r) {
+      const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)word, r);
+      const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(word >> 32), r);
+      const unsigned long long wr = ((unsigned long long)hi << 32) | lo;
+      if (wr == 0ull) continue;  // uniform over the wave
+      const uint32_t off = __builtin_amdgcn_readlane(ex, r);
+      if ((wr >> lane) & 1ull) list[base + off + (uint32_t)__popcll(wr & below)] = (ch * 64u + r) * 64u + lane;
+    }
     if (word) bits[w] = 0ull;
   }
 }
