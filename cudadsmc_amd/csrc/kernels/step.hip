@@ -1171,6 +1171,9 @@ __global__ void k_inject_generate(const InstDesc* __restrict__ Ip, uint32_t opar
   const float twopi = 6.28318530717958647692f;
   const float vm = I.vm_inf;
   const float sd = vm * 0.70710678118654752f;  // sqrt(kT/m)
+  // each particle's rank (the returning count atomic) is stored while the
+  // next particle is drawn, so the atomic's latency overlaps the sampling
+  uint32_t pend_idx = kInvalidKey, pend_rank = 0u;
   for (uint32_t g = s; g < I.n_face_cells; g += I.n_streams) {
     uint32_t off = I.face_count[g], k = I.face_count[g + 1] - off;
     if (k == 0) continue;
@@ -1221,9 +1224,12 @@ __global__ void k_inject_generate(const InstDesc* __restrict__ Ip, uint32_t opar
       I.vx[opar][idx] = v[0]; I.vy[opar][idx] = v[1]; I.vz[opar][idx] = v[2];
       uint32_t key = cell_key_phys(I, x, y, zz);
       I.key[opar][idx] = key;
-      gmem(I.rank)[idx] = gatomic_add(gmem(I.count) + key, 1u);
+      if (pend_idx != kInvalidKey) gmem(I.rank)[pend_idx] = pend_rank;
+      pend_rank = gatomic_add(gmem(I.count) + key, 1u);
+      pend_idx = idx;
     }
   }
+  if (pend_idx != kInvalidKey) gmem(I.rank)[pend_idx] = pend_rank;
   mt_store(I, s, rng);
 }
 
