@@ -128,7 +128,8 @@ struct InstDesc {
   unsigned long long* defer_bits;  // PHYSICS with a body: bit s set = slot s deferred [cap / 64 + 2]
   uint32_t* defer_list; // PHYSICS with a body: deferred slots of this step, compacted [cap]
   uint32_t n_wall_wg;   // PHYSICS with a body: workgroups of the wall pass
-  unsigned long long* wall_fix;  // [4] fixed-point wall sums of the step, [4] workgroups done
+  long long* wall_part;  // PHYSICS with a body: [n_wall_wg][4] each wall-pass workgroup's fixed-point sums
+                        // of the step (folded into wall_acc by one workgroup after the pass)
   double wall_scale[2]; // fixed-point scale of a velocity change (2^36 / v_ref) and of a change of |v|^2
                         // (2^33 / v_ref^2)
   double wall_unit[2];  // value of one fixed-point unit: m v_ref / 2^36 (momentum), m v_ref^2 / 2^34 (energy)

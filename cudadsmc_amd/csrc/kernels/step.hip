@@ -886,14 +886,13 @@ __global__ void __launch_bounds__(kTileThreads) k_defer_list(const InstDesc* __r
 // over the list). The pass holds no RNG state (Philox draws keyed by slot
 // and step). Wall forces and heat are summed in 64-bit fixed point, so the
 // order in which particles are listed and run does not change the sums;
-// the last workgroup to finish converts the step's sums and adds them to
-// wall_acc.
+// each workgroup stores its sums in its own slot of wall_part, and
+// k_wall_sum adds them to wall_acc.
 __global__ void __launch_bounds__(kTileThreads, 4) k_walls(const InstDesc* __restrict__ Ip, uint32_t opar,
                                                        uint32_t flags) {
   const InstDesc& I = Ip[blockIdx.y];
   if (!I.defer_bits || blockIdx.x >= I.n_wall_wg) return;
   __shared__ long long s_red[kTileThreads / 64][4];
-  __shared__ uint32_t s_last;
   const uint32_t cnt = I.ctr[5];
   const uint32_t stepno = I.ctr[4];
   const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / 64;
@@ -937,28 +936,37 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_walls(const InstDesc* __res
     if (lane == 0) s_red[wid][k] = x;
   }
   __syncthreads();
+  if (tid < 4) {  // the workgroup's own slot: plain stores, no contention
+    long long t = 0;
+    for (uint32_t q = 0; q < kTileThreads / 64; ++q) t += s_red[q][tid];
+    gmem(I.wall_part)[blockIdx.x * 4u + tid] = t;
+  }
+}
+
+// One workgroup per instance folds the wall pass's per-workgroup sums into
+// the running totals: exact 64-bit integer sums (any order gives the same
+// value), converted once per step.
+__global__ void __launch_bounds__(kTileThreads) k_wall_sum(const InstDesc* __restrict__ Ip) {
+  const InstDesc& I = Ip[blockIdx.y];
+  if (!I.defer_bits) return;
+  __shared__ long long s_red[kTileThreads / 64][4];
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / 64;
+  const gptr<const long long> part = gmem<const long long>(I.wall_part);
+  long long v[4] = {0, 0, 0, 0};
+  for (uint32_t w = tid; w < I.n_wall_wg; w += kTileThreads)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] += part[w * 4u + k];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    long long x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+    if (lane == 0) s_red[wid][k] = x;
+  }
+  __syncthreads();
   if (tid < 4) {
     long long t = 0;
     for (uint32_t q = 0; q < kTileThreads / 64; ++q) t += s_red[q][tid];
-    if (t) __hip_atomic_fetch_add(gmem(I.wall_fix) + tid, (unsigned long long)t, __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-  if (tid == 0) {
-    __threadfence();
-    const unsigned long long done = __hip_atomic_fetch_add(gmem(I.wall_fix) + 4, 1ull, __ATOMIC_ACQ_REL,
-                                                           __HIP_MEMORY_SCOPE_AGENT);
-    s_last = done + 1ull == (unsigned long long)I.n_wall_wg ? 1u : 0u;
-  }
-  __syncthreads();
-  if (s_last) {
-    // every workgroup's sums are in: convert this step's and start the next
-    if (tid < 4) {
-      const long long t = (long long)__hip_atomic_exchange(gmem(I.wall_fix) + tid, 0ull, __ATOMIC_ACQ_REL,
-                                                           __HIP_MEMORY_SCOPE_AGENT);
-      gmem(I.wall_acc)[tid] += (double)t * I.wall_unit[tid < 3 ? 0 : 1];
-    }
-    if (tid == 0) __hip_atomic_store(gmem(I.wall_fix) + 4, 0ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    gmem(I.wall_acc)[tid] += (double)t * I.wall_unit[tid < 3 ? 0 : 1];
   }
 }
 
@@ -1353,6 +1361,7 @@ void fix_segments(const uint32_t* start, uint32_t n_cells, uint32_t* perm, hipSt
 void walls(const InstDesc* d, uint32_t opar, uint32_t flags, uint32_t grid, hipStream_t st, uint32_t n_inst) {
   hipLaunchKernelGGL(k_defer_list, dim3(grid, n_inst), dim3(kTileThreads), 0, st, d);
   hipLaunchKernelGGL(k_walls, dim3(grid, n_inst), dim3(kTileThreads), 0, st, d, opar, flags);
+  hipLaunchKernelGGL(k_wall_sum, dim3(1, n_inst), dim3(kTileThreads), 0, st, d);
 }
 
 void tile(uint32_t mode, const InstDesc* d, uint32_t par, uint32_t flags, uint32_t n_vwg, hipStream_t st,

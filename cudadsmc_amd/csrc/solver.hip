@@ -597,8 +597,8 @@ static int finish_loading(dsmc_solver* h, uint32_t n, uint32_t* n_out) {
     HIPC(hipMemsetAsync(I.defer_bits, 0, words * 8, st));
     CHECK_RC(h->alloc(&I.defer_list, I.cap));
     I.n_wall_wg = wall_grid(I);
-    CHECK_RC(h->alloc(&I.wall_fix, 8));
-    HIPC(hipMemsetAsync(I.wall_fix, 0, 8 * 8, st));
+    CHECK_RC(h->alloc(&I.wall_part, (size_t)I.n_wall_wg * 4));
+    HIPC(hipMemsetAsync(I.wall_part, 0, (size_t)I.n_wall_wg * 4 * sizeof(long long), st));
     CHECK_RC(h->alloc(&I.wall_acc, 4));
     HIPC(hipMemsetAsync(I.wall_acc, 0, 4 * sizeof(double), st));
     // fixed-point units of the wall sums, from the largest of the speeds
