@@ -840,32 +840,48 @@ template __global__ void k_tile<kModePhysics>(const InstDesc*, uint32_t, uint32_
 // with walls and faces here, in two launches.
 //
 // k_defer_list compacts the marked slots: each wave takes chunks of 64
-// bitmask words (4096 slots), finds their set bits with one coalesced load,
-// reserves room for them in defer_list with one atomic, writes each word's
-// slots as one contiguous run (lane l tests bit l) and clears the words.
+// bitmask words (4096 slots) and finds their set bits with one coalesced
+// load; the workgroup reserves room for its waves' chunks in defer_list with
+// one atomic; each word's slots are written as one contiguous run (lane l
+// tests bit l) and the words cleared.
 __global__ void __launch_bounds__(kTileThreads) k_defer_list(const InstDesc* __restrict__ Ip) {
   const InstDesc& I = Ip[blockIdx.y];
   if (!I.defer_bits || blockIdx.x >= I.n_wall_wg) return;
-  const uint32_t lane = lane_id();
+  __shared__ uint32_t s_tot[kTileThreads / 64u + 1u];
+  const uint32_t lane = lane_id(), wave = threadIdx.x / 64u;
   const uint32_t nwords = (I.ctr[1] + 63u) / 64u;  // deferred slots lie below the sorted count
   const uint32_t nchunks = (nwords + 63u) / 64u;
   const gptr<unsigned long long> bits = gmem(I.defer_bits);
   const gptr<uint32_t> list = gmem(I.defer_list);
-  const uint32_t waves = I.n_wall_wg * (kTileThreads / 64u);
-  for (uint32_t ch = blockIdx.x * (kTileThreads / 64u) + threadIdx.x / 64u; ch < nchunks; ch += waves) {
+  constexpr uint32_t kWaves = kTileThreads / 64u;
+  // the workgroup takes kWaves chunks per round (one per wave) and reserves
+  // room for all of them with one atomic
+  for (uint32_t c0 = blockIdx.x * kWaves; c0 < nchunks; c0 += I.n_wall_wg * kWaves) {
+    const uint32_t ch = c0 + wave;
     const uint32_t w = ch * 64u + lane;
-    const unsigned long long word = w < nwords ? bits[w] : 0ull;
+    const unsigned long long word = (ch < nchunks && w < nwords) ? bits[w] : 0ull;
     const uint32_t pc = (uint32_t)__popcll(word);
     uint32_t incl = pc;
     for (uint32_t off = 1; off < 64u; off <<= 1) {
       const uint32_t t = __shfl_up(incl, off);
       if (lane >= off) incl += t;
     }
-    const uint32_t total = __shfl(incl, 63);
-    if (total == 0u) continue;  // uniform over the wave
-    uint32_t base = 0u;
-    if (lane == 0) base = gatomic_add(gmem(I.ctr) + 5, total);
-    base = __shfl(base, 0);
+    if (lane == 63u) s_tot[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t sum = 0;
+      for (uint32_t q = 0; q < kWaves; ++q) {
+        const uint32_t x = s_tot[q];
+        s_tot[q] = sum;
+        sum += x;
+      }
+      s_tot[kWaves] = sum ? gatomic_add(gmem(I.ctr) + 5, sum) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = s_tot[kWaves] + s_tot[wave];
+    const bool any = __shfl(incl, 63) != 0u;
+    __syncthreads();  // s_tot is rewritten in the next round
+    if (!any) continue;  // uniform over the wave
     // word r of the chunk (read from lane r): each lane tests its own bit,
     // so the word's set slots are written as one contiguous run
     const uint32_t ex = incl - pc;
