@@ -1154,28 +1154,35 @@ __device__ __forceinline__ uint32_t face_of(const InstDesc& I, uint32_t g) {
   return f;
 }
 
+// Injection draws Philox4x32-10 under key (seed, kInjectKey1), so no stream
+// state moves and every particle is independent: the count of face cell g
+// draws counter (g, step, 0, 0); particle j of g draws (g, step, 1 + j, b)
+// for its b-th rejection block (two tries per block) and (g, step, 1 + j,
+// 2^31 + k) for its other variates.
+constexpr uint32_t kInjectKey1 = 0xBB67AE85u;
+
 __global__ void k_inject_count(const InstDesc* __restrict__ Ip) {
   const InstDesc& I = Ip[blockIdx.y];
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= I.n_streams) return;
-  if (s >= I.n_face_cells) return;
-  Mt rng;
-  mt_load(I, s, rng);
-  for (uint32_t g = s; g < I.n_face_cells; g += I.n_streams) {
-    uint32_t f = face_of(I, g);
-    float lam = I.inj_lambda[f];
-    I.face_count[g] = dm_f2u_sat(lam + mt_float(rng));
+  const uint32_t stepno = I.ctr[4];
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < I.n_face_cells; g += gridDim.x * blockDim.x) {
+    const dsmc_rng::U32x4 c = {{g, stepno, 0u, 0u}};
+    const dsmc_rng::U32x4 r = dsmc_rng::philox4x32_10(c, I.wall_key[0], kInjectKey1);
+    I.face_count[g] = dm_f2u_sat(I.inj_lambda[face_of(I, g)] + dsmc_rng::px_unit(r.v[0]));
   }
-  mt_store(I, s, rng);
 }
 
+// One thread per injected particle t (grid-stride): its face cell is the
+// last g with face_count[g] <= t (face_count holds the exclusive offsets),
+// it is particle t - face_count[g] of that cell, and it goes to slot
+// sorted count + t.
 __global__ void k_inject_generate(const InstDesc* __restrict__ Ip, uint32_t opar) {
   const InstDesc& I = Ip[blockIdx.y];
-  if (I.n_face_cells == 0) return;
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t F = I.n_face_cells;
+  if (F == 0) return;
+  const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t base = I.ctr[1];
-  if (s == 0) {
-    uint32_t tot = I.face_count[I.n_face_cells];
+  const uint32_t tot = I.face_count[F];
+  if (t0 == 0) {
     uint32_t end = base + tot;
     if (end > I.cap || end < base) {  // capacity exceeded: drop the excess, flag it
       end = I.cap;
@@ -1184,77 +1191,76 @@ __global__ void k_inject_generate(const InstDesc* __restrict__ Ip, uint32_t opar
     I.ctr[0] = end;
     I.ctr[2] += end - base;
   }
-  if (s >= I.n_streams || s >= I.n_face_cells) return;
-  // most face cells inject nothing in a step (outflow and side faces): such
-  // threads draw nothing and leave their stream untouched
-  bool any = false;
-  for (uint32_t g = s; g < I.n_face_cells && !any; g += I.n_streams) any = I.face_count[g + 1] > I.face_count[g];
-  if (!any) return;
-  Mt rng;
-  mt_load(I, s, rng);
+  const uint32_t stepno = I.ctr[4];
+  const uint32_t k0 = I.wall_key[0];
+  const gptr<const uint32_t> fc = gmem<const uint32_t>(I.face_count);
   const float twopi = 6.28318530717958647692f;
   const float vm = I.vm_inf;
   const float sd = vm * 0.70710678118654752f;  // sqrt(kT/m)
-  // each particle's rank (the returning count atomic) is stored while the
-  // next particle is drawn, so the atomic's latency overlaps the sampling
-  uint32_t pend_idx = kInvalidKey, pend_rank = 0u;
-  for (uint32_t g = s; g < I.n_face_cells; g += I.n_streams) {
-    uint32_t off = I.face_count[g], k = I.face_count[g + 1] - off;
-    if (k == 0) continue;
-    uint32_t f = face_of(I, g);
-    uint32_t local = g - I.face_off[f];
-    uint32_t axis = f >> 1;           // normal axis
-    uint32_t ta = axis == 0 ? 1u : 0u;  // first tangential axis
-    uint32_t tb = axis == 2 ? 1u : 2u;  // second tangential axis
-    uint32_t na = I.dim[ta];
-    uint32_t ia = local % na, ib = local / na;
-    float sign = (f & 1u) ? -1.0f : 1.0f;  // inward normal direction
-    float un = sign * I.U[axis];           // inward stream component
-    float sr = un / vm;                    // speed ratio of the inflow
+  for (uint32_t t = t0; t < tot; t += gridDim.x * blockDim.x) {
+    const uint32_t idx = base + t;
+    if (idx >= I.cap || idx < base) break;  // later t only go further
+    uint32_t lo = 0u, hi = F;  // fc[lo] <= t < fc[hi]
+    while (hi - lo > 1u) {
+      const uint32_t mid = lo + (hi - lo) / 2u;
+      if (fc[mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t g = lo, j = t - fc[g];
+    const uint32_t f = face_of(I, g);
+    const uint32_t local = g - I.face_off[f];
+    const uint32_t axis = f >> 1;           // normal axis
+    const uint32_t ta = axis == 0 ? 1u : 0u;  // first tangential axis
+    const uint32_t tb = axis == 2 ? 1u : 2u;  // second tangential axis
+    const uint32_t na = I.dim[ta];
+    const uint32_t ia = local % na, ib = local / na;
+    const float sign = (f & 1u) ? -1.0f : 1.0f;  // inward normal direction
+    const float un = sign * I.U[axis];           // inward stream component
+    const float sr = un / vm;                    // speed ratio of the inflow
     float zlo = sr - 5.0f;
     zlo = zlo > 0.0f ? zlo : 0.0f;
     float zhi = sr + 5.0f;
     zhi = zhi > 1.0e-3f ? zhi : 1.0e-3f;
     float zs = 0.5f * (sr + sqrtf(sr * sr + 2.0f));  // mode of z exp(-(z-s)^2)
     zs = zs < zlo ? zlo : (zs > zhi ? zhi : zs);
-    float gmax = zs * dm_expf(-(zs - sr) * (zs - sr));
-    for (uint32_t j = 0; j < k; ++j) {
-      float z;
-      for (int it = 0; it < 1000; ++it) {
-        z = zlo + (zhi - zlo) * mt_float(rng);
-        float gz = z * dm_expf(-(z - sr) * (z - sr));
-        if (gz > gmax * mt_float(rng)) break;
+    const float gmax = zs * dm_expf(-(zs - sr) * (zs - sr));
+    float z = zlo;
+    bool ok = false;
+    for (uint32_t b = 0; b < 500u && !ok; ++b) {
+      const dsmc_rng::U32x4 c = {{g, stepno, 1u + j, b}};
+      const dsmc_rng::U32x4 r = dsmc_rng::philox4x32_10(c, k0, kInjectKey1);
+      for (int h = 0; h < 2 && !ok; ++h) {
+        z = zlo + (zhi - zlo) * dsmc_rng::px_unit(r.v[2 * h]);
+        const float gz = z * dm_expf(-(z - sr) * (z - sr));
+        ok = gz > gmax * dsmc_rng::px_unit(r.v[2 * h + 1]);
       }
-      float vn = z * vm;
-      float r1 = mt_float(rng), r2 = mt_float(rng);
-      float a1 = sqrtf(-2.0f * dm_logf(r1));
-      float s1, c1;
-      dm_sincosf(twopi * r2, &s1, &c1);
-      float p[3], v[3];
-      v[axis] = sign * vn;
-      v[ta] = I.U[ta] + sd * a1 * c1;
-      v[tb] = I.U[tb] + sd * a1 * s1;
-      p[axis] = (f & 1u) ? I.gmax[axis] : I.gmin[axis];
-      p[ta] = I.gmin[ta] + ((float)ia + mt_float(rng)) * I.cell[ta];
-      p[tb] = I.gmin[tb] + ((float)ib + mt_float(rng)) * I.cell[tb];
-      float fr = mt_float(rng) * I.dt;
-      float x = p[0] + v[0] * fr, y = p[1] + v[1] * fr, zz = p[2] + v[2] * fr;
-      x = x < I.gmin[0] ? I.gmin[0] : (x > I.gmax[0] ? I.gmax[0] : x);
-      y = y < I.gmin[1] ? I.gmin[1] : (y > I.gmax[1] ? I.gmax[1] : y);
-      zz = zz < I.gmin[2] ? I.gmin[2] : (zz > I.gmax[2] ? I.gmax[2] : zz);
-      uint32_t idx = base + off + j;
-      if (idx >= I.cap) continue;
-      I.px[opar][idx] = x; I.py[opar][idx] = y; I.pz[opar][idx] = zz;
-      I.vx[opar][idx] = v[0]; I.vy[opar][idx] = v[1]; I.vz[opar][idx] = v[2];
-      uint32_t key = cell_key_phys(I, x, y, zz);
-      I.key[opar][idx] = key;
-      if (pend_idx != kInvalidKey) gmem(I.rank)[pend_idx] = pend_rank;
-      pend_rank = gatomic_add(gmem(I.count) + key, 1u);
-      pend_idx = idx;
     }
+    const dsmc_rng::U32x4 ca = {{g, stepno, 1u + j, 0x80000000u}};
+    const dsmc_rng::U32x4 ra = dsmc_rng::philox4x32_10(ca, k0, kInjectKey1);
+    const dsmc_rng::U32x4 cb = {{g, stepno, 1u + j, 0x80000001u}};
+    const dsmc_rng::U32x4 rb = dsmc_rng::philox4x32_10(cb, k0, kInjectKey1);
+    const float vn = z * vm;
+    const float a1 = sqrtf(-2.0f * dm_logf(dsmc_rng::px_unit(ra.v[0])));
+    float s1, c1;
+    dm_sincosf(twopi * dsmc_rng::px_unit(ra.v[1]), &s1, &c1);
+    float p[3], v[3];
+    v[axis] = sign * vn;
+    v[ta] = I.U[ta] + sd * a1 * c1;
+    v[tb] = I.U[tb] + sd * a1 * s1;
+    p[axis] = (f & 1u) ? I.gmax[axis] : I.gmin[axis];
+    p[ta] = I.gmin[ta] + ((float)ia + dsmc_rng::px_unit(ra.v[2])) * I.cell[ta];
+    p[tb] = I.gmin[tb] + ((float)ib + dsmc_rng::px_unit(ra.v[3])) * I.cell[tb];
+    const float fr = dsmc_rng::px_unit(rb.v[0]) * I.dt;
+    float x = p[0] + v[0] * fr, y = p[1] + v[1] * fr, zz = p[2] + v[2] * fr;
+    x = x < I.gmin[0] ? I.gmin[0] : (x > I.gmax[0] ? I.gmax[0] : x);
+    y = y < I.gmin[1] ? I.gmin[1] : (y > I.gmax[1] ? I.gmax[1] : y);
+    zz = zz < I.gmin[2] ? I.gmin[2] : (zz > I.gmax[2] ? I.gmax[2] : zz);
+    I.px[opar][idx] = x; I.py[opar][idx] = y; I.pz[opar][idx] = zz;
+    I.vx[opar][idx] = v[0]; I.vy[opar][idx] = v[1]; I.vz[opar][idx] = v[2];
+    const uint32_t key = cell_key_phys(I, x, y, zz);
+    I.key[opar][idx] = key;
+    gmem(I.rank)[idx] = gatomic_add(gmem(I.count) + key, 1u);
   }
-  if (pend_idx != kInvalidKey) gmem(I.rank)[pend_idx] = pend_rank;
-  mt_store(I, s, rng);
 }
 
 // ================================================================== utility

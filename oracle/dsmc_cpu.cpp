@@ -30,8 +30,9 @@
 // tile count; else workgroup w takes positions w, w + G, ...
 // Cell lane l of a tile draws from stream w * threads + l and
 // the move of tile position q from stream w * threads + q mod threads;
-// loading and injection draw from stream (item mod n_streams). The oracle
-// follows that assignment with plain loops.
+// loading draws from stream (item mod n_streams); injection and the wall
+// re-emission draw counter-based Philox4x32-10. The oracle follows that
+// assignment with plain loops.
 #include "dsmc_cpu.h"
 
 #include <algorithm>
@@ -770,7 +771,16 @@ struct oracle_sim {
     if (!F) return;
     std::vector<uint32_t> cnt(F);
     auto face_of = [&](uint32_t g) { uint32_t f = 0; while (f < 5 && g >= off[f + 1]) ++f; return f; };
-    for (uint32_t g = 0; g < F; ++g) cnt[g] = dm_f2u_sat(inj_lambda[face_of(g)] + rng[g % S].uniform());
+    // Philox4x32-10 under key (seed, kInjectKey1): the count of face cell g
+    // at counter (g, step, 0, 0); particle j of g at (g, step, 1 + j, b) for
+    // rejection block b (two tries each) and (g, step, 1 + j, 2^31 + k)
+    const uint32_t k0 = (uint32_t)s.seed, k1 = 0xBB67AE85u;
+    auto draw = [&](uint32_t c0, uint32_t c2, uint32_t c3) {
+      dsmc_rng::U32x4 c = {{c0, stepno, c2, c3}};
+      return dsmc_rng::philox4x32_10(c, k0, k1);
+    };
+    for (uint32_t g = 0; g < F; ++g)
+      cnt[g] = dm_f2u_sat(inj_lambda[face_of(g)] + dsmc_rng::px_unit(draw(g, 0u, 0u).v[0]));
     std::vector<uint32_t> start(F + 1, 0);
     for (uint32_t g = 0; g < F; ++g) start[g + 1] = start[g] + cnt[g];
     const uint32_t base = n_store, tot = start[F];
@@ -784,7 +794,6 @@ struct oracle_sim {
     for (uint32_t g = 0; g < F; ++g) {
       uint32_t k = cnt[g];
       if (!k) continue;
-      MtBlock& r = rng[g % S];
       uint32_t f = face_of(g), local = g - off[f];
       uint32_t axis = f >> 1, ta = axis == 0 ? 1u : 0u, tb = axis == 2 ? 1u : 2u;
       uint32_t ia = local % dims[ta], ib = local / dims[ta];
@@ -799,25 +808,29 @@ struct oracle_sim {
       zs = zs < zlo ? zlo : (zs > zhi ? zhi : zs);
       float gmx = zs * dm_expf(-(zs - sr) * (zs - sr));
       for (uint32_t j = 0; j < k; ++j) {
-        float zz = 0.0f;
-        for (int it = 0; it < 1000; ++it) {
-          zz = zlo + (zhi - zlo) * r.uniform();
-          float gz = zz * dm_expf(-(zz - sr) * (zz - sr));
-          if (gz > gmx * r.uniform()) break;
+        float zz = zlo;
+        bool ok = false;
+        for (uint32_t b = 0; b < 500u && !ok; ++b) {
+          const dsmc_rng::U32x4 rr = draw(g, 1u + j, b);
+          for (int h = 0; h < 2 && !ok; ++h) {
+            zz = zlo + (zhi - zlo) * dsmc_rng::px_unit(rr.v[2 * h]);
+            float gz = zz * dm_expf(-(zz - sr) * (zz - sr));
+            ok = gz > gmx * dsmc_rng::px_unit(rr.v[2 * h + 1]);
+          }
         }
+        const dsmc_rng::U32x4 ra = draw(g, 1u + j, 0x80000000u), rb = draw(g, 1u + j, 0x80000001u);
         float vn = zz * vm_inf;
-        float r1 = r.uniform(), r2 = r.uniform();
-        float a1 = std::sqrt(-2.0f * dm_logf(r1));
+        float a1 = std::sqrt(-2.0f * dm_logf(dsmc_rng::px_unit(ra.v[0])));
         float s1, c1;
-        dm_sincosf(twopi * r2, &s1, &c1);
+        dm_sincosf(twopi * dsmc_rng::px_unit(ra.v[1]), &s1, &c1);
         float p[3], v[3];
         v[axis] = sign * vn;
         v[ta] = U[ta] + sd * a1 * c1;
         v[tb] = U[tb] + sd * a1 * s1;
         p[axis] = (f & 1u) ? gmax[axis] : gmin[axis];
-        p[ta] = gmin[ta] + ((float)ia + r.uniform()) * dims_cell[ta];
-        p[tb] = gmin[tb] + ((float)ib + r.uniform()) * dims_cell[tb];
-        float fr = r.uniform() * dt;
+        p[ta] = gmin[ta] + ((float)ia + dsmc_rng::px_unit(ra.v[2])) * dims_cell[ta];
+        p[tb] = gmin[tb] + ((float)ib + dsmc_rng::px_unit(ra.v[3])) * dims_cell[tb];
+        float fr = dsmc_rng::px_unit(rb.v[0]) * dt;
         float px = p[0] + v[0] * fr, py = p[1] + v[1] * fr, pz = p[2] + v[2] * fr;
         px = px < gmin[0] ? gmin[0] : (px > gmax[0] ? gmax[0] : px);
         py = py < gmin[1] ? gmin[1] : (py > gmax[1] ? gmax[1] : py);
