@@ -62,6 +62,7 @@ def tune_solver(make: Callable[[int], "object"], streams: Iterable[int] = DEFAUL
         dt = sol.settings.dt
         t = _time_steps(lambda k: sol.DoBenchmark(k, dt), sol.synchronize, warmup, steps)
         table.append({"n_streams": int(ns), "scatter_grid": None, "ms_per_step": 1e3 * t})
+        print("solver", json.dumps(table[-1]), flush=True)
         if best is None or t < best[0]:
             best = (t, int(ns))
         sol.close()
@@ -72,6 +73,7 @@ def tune_solver(make: Callable[[int], "object"], streams: Iterable[int] = DEFAUL
         sol.set_scatter_grid(int(g))
         t = _time_steps(lambda k: sol.DoBenchmark(k, dt), sol.synchronize, warmup // 2 + 2, steps)
         table.append({"n_streams": best[1], "scatter_grid": int(g), "ms_per_step": 1e3 * t})
+        print("solver", json.dumps(table[-1]), flush=True)
         if sbest is None or t < sbest[0]:
             sbest = (t, int(g))
     sol.close()
@@ -162,6 +164,7 @@ def main(argv=None) -> None:
                 t = _time_steps(lambda k: sol.DoBenchmark(k, dt), sol.synchronize, 600, 40)
                 sol.close()
                 rtab.append({"cg_refine": refine, "ms_per_step": 1e3 * t})
+                print("c3", json.dumps(rtab[-1]), flush=True)
             refine = min(rtab, key=lambda r: r["ms_per_step"])["cg_refine"]
             res = tune_solver(lambda ns: make(ns, refine), streams=(262144, 524288, 1048576, 2097152),
                               scatter=(1024, 2048, 4096, 8192), warmup=600, steps=40)
