@@ -468,7 +468,7 @@ using dsmc_launch::kFlagSample;
 using dsmc_launch::kFlagKeepPerm;
 using dsmc_launch::kFlagHits;
 
-constexpr uint32_t kMoveBatch = 4;  // rounds of 256 positions whose loads are issued together
+constexpr uint32_t kMoveBatch = 6;  // rounds of 256 positions whose loads are issued together
 // PHYSICS: rounds of 256 permutation entries of the next tile loaded into
 // registers during this tile's move (the next tile's first phase then only
 // stores them to LDS)
