@@ -39,46 +39,74 @@ TILE_B_P, TILE_B_C = 56.0, 20.0
 
 METRIC = "particle-steps/s + %HBM peak on 1 MI355X; ensemble time-to-steady warm vs cold"
 
-WORKLOADS = {
-    # name: (grid, target particles, n, U, dt, body)
-    "c3": dict(grid=(256, 128, 128), n_part=2**25, density=1.0e21, U=3000.0, dt=2.0e-7, body="spherecone"),
-    "c2": dict(grid=(128, 64, 64), n_part=2**20, density=1.0e20, U=1500.0, dt=5.0e-7, body="ico5"),
-    "c1": dict(grid=(64, 32, 32), n_part=100000, density=1.0e20, U=1500.0, dt=1.0e-6, body="ico3"),
-}
+from cudadsmc_amd.cases import WORKLOADS, make_case  # noqa: E402
 
 
-def make_case(name: str, n_part: int | None = None, cg_refine: int | None = None):
-    """Settings, body mesh and collision-grid dimensions of a workload. The
-    collision grid covers the body's bounding box with cells `cg_refine`
-    times finer than the flow cells (default: the tuned value, else 1)."""
+def make_solver_for(name: str, cg_refine: int | None = None, tile_cells: int | None = None,
+                    scatter_grid: int | None = None):
+    """A solver of workload `name` with the tuned launch layout (RNG streams,
+    cells per tile, scatter grid), geometry set, no particles loaded."""
     import cudadsmc_amd as dm
     from cudadsmc_amd import ensemble as ens
 
-    w = WORKLOADS[name]
-    s = dm.settings_default()
-    s.mode = dm.MODE_PHYSICS
-    s.bmin[:] = (0.0, 0.0, 0.0)
-    s.bmax[:] = (0.4, 0.2, 0.2)
-    s.grid_dim[:] = w["grid"]
-    for a in range(3):
-        s.cell_size[a] = (s.bmax[a] - s.bmin[a]) / s.grid_dim[a]
-    s.molmass, s.diameter, s.omega, s.Tref = 6.63e-26, 4.17e-10, 0.81, 273.0
-    s.temp, s.density, s.wall_temp = 200.0, w["density"], 300.0
-    s.stream_velocity[:] = (w["U"], 0.0, 0.0)
-    s.face_bc[:] = [dm.FACE_FREESTREAM] * 6
-    s.target_particles = n_part or w["n_part"]
-    s.dt = w["dt"]
-    if w["body"] == "spherecone":
-        mesh = dm.sphere_cone(0.1, 0.1, 0.1, 0.02, 30.0, 0.05, 200, 124)
-    elif w["body"] == "ico5":
-        mesh = dm.icosphere((0.1, 0.1, 0.1), 0.025, 5)
-    else:
-        mesh = dm.icosphere((0.1, 0.1, 0.1), 0.025, 3)
-    if cg_refine is None:
-        cg_refine = int(ens.tuned(name, 1, "cg_refine"))
-    lo, hi = mesh.vertices.min(0), mesh.vertices.max(0)
-    cgrid = tuple(int(max(1, np.ceil(cg_refine * (hi[k] - lo[k]) / s.cell_size[k]))) for k in range(3))
-    return s, mesh, cgrid
+    s, mesh, cgrid = make_case(name, cg_refine=cg_refine)
+    s.n_streams = int(ens.tuned(name, 0))  # 0: the engine default
+    sol = dm.Solver()
+    sol.Init(s)
+    tc = tile_cells or int(ens.tuned(name, 0, "tile_cells"))
+    if tc:
+        sol.set_tile_cells(tc)
+    sg = scatter_grid or int(ens.tuned(name, 0, "scatter_grid"))
+    if sg:
+        sol.set_scatter_grid(sg)
+    sol.SetModel(mesh, grid=cgrid)
+    return sol, s, mesh
+
+
+def timed(sol, steps: int, dt: float) -> dict:
+    """`steps` steps with per-kernel HIP-event timing: wall time, particles
+    (mean of before and after) and the kernel classes' average ms."""
+    import ctypes as C
+
+    import cudadsmc_amd as dm
+
+    _, n_before, _ = sol.counts()
+    sol.set_timing(True)
+    sol.synchronize()
+    t0 = time.perf_counter()
+    sol.DoBenchmark(steps, dt)
+    sol.synchronize()
+    el = time.perf_counter() - t0
+    tm = np.zeros(7, np.float32)
+    dm.check(dm.lib().dsmc_get_timings(sol._h, tm.ctypes.data_as(C.POINTER(C.c_float)), 7), sol._h)
+    sol.set_timing(False)
+    _, n_after, _ = sol.counts()
+    return {"elapsed": el, "n_avg": 0.5 * (n_before + n_after), "tm": tm}
+
+
+def throughput_line(sol, s, steps: int, dt: float, sampling: bool = False) -> dict:
+    r = timed(sol, steps, dt)
+    cells = int(np.prod(s.grid_dim))
+    bp, bc = (B_P_SAMPLE, B_C_SAMPLE) if sampling else (B_P, B_C)
+    tile_ms = float(r["tm"][2])
+    tile_bytes = TILE_B_P * r["n_avg"] + TILE_B_C * cells
+    return {"steps": steps, "ms_per_step": 1e3 * r["elapsed"] / steps,
+            "particle_steps_per_s": r["n_avg"] * steps / r["elapsed"], "particles_avg": r["n_avg"],
+            "pct_hbm_peak": 100.0 * (bp * r["n_avg"] + bc * cells) * steps / r["elapsed"] / HBM_PEAK,
+            "k_tile_ms": tile_ms, "k_tile_frac": tile_bytes / (tile_ms * 1e-3) / HBM_PEAK,
+            "k_walls_ms": float(r["tm"][6])}
+
+
+def other_workload(name: str, warmup: int, steps: int) -> dict:
+    sol, s, mesh = make_solver_for(name)
+    n0 = sol.InitSystemState(tuple(s.stream_velocity))
+    sol.DoBenchmark(warmup, s.dt)
+    sol.synchronize()
+    out = throughput_line(sol, s, steps, s.dt)
+    out.update({"particles_loaded": n0, "warmup": warmup, "grid": list(s.grid_dim),
+                "triangles": int(len(mesh.triangles))})
+    sol.close()
+    return out
 
 
 def measured_traffic(workload: str):
@@ -95,43 +123,54 @@ def measured_traffic(workload: str):
     return None
 
 
-def cpu_baseline(seconds: float = 15.0) -> dict:
-    """The CPU oracle on a bounded sample of the same workload: the full c3 case
-    (2^25 particles, same grid, body and flow), as many steps as fit in
-    `seconds` after loading (at most 50)."""
+def _oracle_case(name: str):
+    """The oracle mirrored from a solver of workload `name` created on the
+    GPU (derived scalars, collision grid, RNG layout; no particles there)."""
     import oracle
 
-    import cudadsmc_amd as dm
-
-    s, mesh, cgrid = make_case("c3")
-    # the oracle needs the engine's derived scalars and RNG layout: mirror a
-    # solver created on the GPU (no particles loaded there)
-    sol = dm.Solver()
-    sol.Init(s)
-    sol.SetModel(mesh, grid=cgrid)
+    sol, s, mesh = make_solver_for(name)
     o = oracle.OracleSim.from_solver(sol)
     o.set_geometry(mesh, sol.collision_grid(), sol.fluid_fraction())
     o.set_near_cells(sol.near_cells())
     o.set_rng(sol.rng_state())
     o.set_ntc(sol.ntc_state())
     sol.close()
-    o.load(tuple(s.stream_velocity))
-    n0 = o.state()["pos"].shape[1]
-    steps, t0 = 0, time.perf_counter()
-    psteps = 0
-    while True:
-        n = int(o.cell_start()[-1]) if steps else n0
-        o.step(sample=False)
-        psteps += n
-        steps += 1
-        if time.perf_counter() - t0 > seconds or steps >= 50:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": psteps / dt, "unit": "particle-steps/s", "cores": int(oracle.lib().oracle_threads()),
-            "kind": "port",
-            "sample": f"oracle (dsmc_cpu.cpp, OpenMP) on the c3 workload ({n0} particles loaded), "
-                      f"{steps} steps after loading ({dt:.1f} s)",
-            "cpu_model": _cpu_model()}
+    return o, s
+
+
+def cpu_baseline(name: str = "c3", seconds: float = 15.0, max_steps: int = 50, threads: int = 0,
+                 sampling_from: int | None = None) -> dict:
+    """The CPU oracle on workload `name`: after loading, steps until `seconds`
+    have passed or `max_steps` are done (steps from `sampling_from` on are
+    sampled); `threads` > 0 sets the OpenMP thread count."""
+    import oracle
+
+    L = oracle.lib()
+    t_all = L.oracle_threads()
+    if threads:
+        L.oracle_set_threads(threads)
+    try:
+        o, s = _oracle_case(name)
+        o.load(tuple(s.stream_velocity))
+        n0 = o.state()["pos"].shape[1]
+        steps, psteps, t0 = 0, 0, time.perf_counter()
+        while True:
+            n = int(o.cell_start()[-1]) if steps else n0
+            o.step(sample=sampling_from is not None and steps >= sampling_from)
+            psteps += n
+            steps += 1
+            if time.perf_counter() - t0 > seconds or steps >= max_steps:
+                break
+        dt = time.perf_counter() - t0
+        cores = int(L.oracle_threads())
+    finally:
+        L.oracle_set_threads(t_all)
+    done = "to completion" if steps >= max_steps else "bounded sample"
+    return {"value": psteps / dt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"oracle (dsmc_cpu.cpp, OpenMP, {cores} threads) on the {name} workload ({n0} particles "
+                      f"loaded), {steps} steps after loading ({dt:.1f} s, {done})",
+            "cpu_model": _cpu_model(), "hardware_concurrency": os.cpu_count(),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
 
 
 def ensemble_bench(steps: int, warmup: int, instances: int = 64) -> dict:
@@ -194,6 +233,12 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized buffer 5 times (calibrates HBM byte counters)")
     ap.add_argument("--ensemble-steps", type=int, default=300)
+    ap.add_argument("--no-developed", action="store_true",
+                    help="skip the developed-flow and sampling-step measurements")
+    ap.add_argument("--developed-warmup", type=int, default=1000,
+                    help="steps after loading before the developed-flow measurement")
+    ap.add_argument("--developed-steps", type=int, default=100)
+    ap.add_argument("--no-other-workloads", action="store_true", help="skip the c1 and c2 throughput lines")
     ap.add_argument("--scatter-grid", type=int, default=None,
                     help="workgroups of the scatter kernel (default: tuned, else the engine's)")
     ap.add_argument("--cg-refine", type=int, default=None,
@@ -224,21 +269,10 @@ def main() -> int:
                           "copy_GBps": 2 * 4 * n / (ms * 1e-3) / 1e9}))
         return 0
 
-    import cudadsmc_amd as dm
-    from cudadsmc_amd import ensemble as ens
-
-    s, mesh, cgrid = make_case(args.workload, cg_refine=args.cg_refine)
-    s.n_streams = int(ens.tuned(args.workload, 0))  # 0: the engine default
     t_setup = time.perf_counter()
-    sol = dm.Solver()
-    sol.Init(s)
-    tc = args.tile_cells or int(ens.tuned(args.workload, 0, "tile_cells"))
-    if tc:
-        sol.set_tile_cells(tc)
-    sg = args.scatter_grid or int(ens.tuned(args.workload, 0, "scatter_grid"))
-    if sg:
-        sol.set_scatter_grid(sg)
-    sol.SetModel(mesh, grid=cgrid)
+    sol, s, mesh = make_solver_for(args.workload, cg_refine=args.cg_refine, tile_cells=args.tile_cells,
+                                   scatter_grid=args.scatter_grid)
+    cgrid = [int(x) for x in sol.collision_grid()["dims"]]
     n0 = sol.InitSystemState(tuple(s.stream_velocity))
     sol.synchronize()
     t_setup = time.perf_counter() - t_setup
@@ -248,26 +282,14 @@ def main() -> int:
     sol.DoBenchmark(args.warmup, s.dt)
     sol.synchronize()
     print(f"[bench] warmup {args.warmup} steps in {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
-
-    _, n_before, _ = sol.counts()
-    sol.set_timing(True)
     try:
         import torch
 
         torch.cuda.synchronize()
     except Exception:  # noqa: BLE001
         pass
-    t0 = time.perf_counter()
-    sol.DoBenchmark(args.steps, s.dt)
-    sol.synchronize()
-    elapsed = time.perf_counter() - t0
-    tm = np.zeros(7, np.float32)
-    import ctypes as C
-
-    dm.check(dm.lib().dsmc_get_timings(sol._h, tm.ctypes.data_as(C.POINTER(C.c_float)), 7), sol._h)
-    sol.set_timing(False)
-    _, n_after, _ = sol.counts()
-    n_avg = 0.5 * (n_before + n_after)
+    r = timed(sol, args.steps, s.dt)
+    elapsed, n_avg, tm = r["elapsed"], r["n_avg"], r["tm"]
     psteps = n_avg * args.steps / elapsed
     cells = int(np.prod(s.grid_dim))
     step_bytes = B_P * n_avg + B_C * cells
@@ -299,6 +321,8 @@ def main() -> int:
             "tile_cells": int(sol.layout()["tile_cells"]),
             "parallelism": "single GPU",
         },
+        "flow": f"timed steps {args.warmup + 1}-{args.warmup + args.steps} after a uniform freestream start "
+                f"(the shock layer and wake form over the first ~1000 steps; see developed_flow)",
         "pct_hbm_peak": 100.0 * hbm_step / HBM_PEAK,
         "hbm_model": "144 B/particle-step + 32 B/cell-step (algorithmic, whole step)",
         "kernel_ms_per_step": {"scan": float(tm[0]), "scatter": float(tm[1]), "tile": float(tm[2]),
@@ -306,17 +330,46 @@ def main() -> int:
                                "timed_steps": int(tm[5])},
         "roofline": {"bound": "hbm", "achieved": tile_gbs / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": tile_gbs / HBM_PEAK, "traffic": measured_traffic(args.workload),
+                     "traffic_source": f"stored: profiles/k_tile_{args.workload}_traffic.txt (rocprofv3 --pmc "
+                                       "FETCH_SIZE + WRITE_SIZE per k_tile launch, developed flow, copy-calibrated)",
                      "kernel": "k_tile (56 B/particle + 20 B/cell algorithmic)"},
         "validate_bad_particles": int(bad),
     }
+    if not args.no_developed:
+        # the same solver continued into the developed flow, then timed again
+        done = args.warmup + args.steps
+        if done < args.developed_warmup:
+            sol.DoBenchmark(args.developed_warmup - done, s.dt)
+            sol.synchronize()
+        dev = throughput_line(sol, s, args.developed_steps, s.dt)
+        dev["after_steps"] = max(done, args.developed_warmup)
+        line["developed_flow"] = dev
+        sol.set_sampling(True)
+        smp = throughput_line(sol, s, max(10, args.developed_steps // 2), s.dt, sampling=True)
+        sol.set_sampling(False)
+        smp["hbm_model"] = "156 B/particle-step + 120 B/cell-step (sampling steps, FP64 accumulators)"
+        line["sampling_steps"] = smp
+        print(f"[bench] developed flow {dev['ms_per_step']:.3f} ms/step, sampling {smp['ms_per_step']:.3f} ms/step",
+              file=sys.stderr, flush=True)
     sol.close()
     if not args.no_ensemble:
         t0 = time.perf_counter()
         line["ensemble"] = ensemble_bench(args.ensemble_steps, 500)
         print(f"[bench] ensemble {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    if not args.no_other_workloads and args.workload == "c3":
+        t0 = time.perf_counter()
+        line["other_workloads"] = {"c1": other_workload("c1", 1000, 1000), "c2": other_workload("c2", 1000, 500)}
+        print(f"[bench] c1/c2 {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
-        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
+        if args.workload == "c3":
+            line["cpu_baseline_more"] = {
+                # c1 to completion: 1000 transient + 1000 sampled steps
+                "c1_full": cpu_baseline("c1", 600.0, 2000, sampling_from=1000),
+                "c1_one_thread": cpu_baseline("c1", 10.0, 2000, threads=1),
+                "c3_one_thread": cpu_baseline("c3", 10.0, 50, threads=1),
+            }
         print(f"[bench] cpu baseline {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     print(json.dumps(line))
     return 0

@@ -138,11 +138,19 @@ class Solver:
         self._c(lib().dsmc_init(self._h, C.byref(s)))
         self.settings = s
 
-    def SetModel(self, mesh: Mesh, grid=(5, 5, 20)) -> None:
+    def SetModel(self, mesh: Mesh, grid=(5, 5, 20), move=None) -> None:
+        """SetModel with the collision grid of `grid` cells; `move` shifts the
+        mesh and its grid after the grid is built (Model::CreateGrid then
+        Model::MoveModel, as the reference's driver does)."""
         v = np.ascontiguousarray(mesh.vertices, np.float32)
         t = np.ascontiguousarray(mesh.triangles, np.uint32)
         n = np.ascontiguousarray(mesh.normals, np.float32)
-        self._c(lib().dsmc_set_geometry(self._h, _fp(v), len(v), _up(t), len(t), _fp(n), *map(int, grid)))
+        if move is None:
+            self._c(lib().dsmc_set_geometry(self._h, _fp(v), len(v), _up(t), len(t), _fp(n), *map(int, grid)))
+        else:
+            mv = np.ascontiguousarray(move, np.float32)
+            self._c(lib().dsmc_set_geometry_moved(self._h, _fp(v), len(v), _up(t), len(t), _fp(n), *map(int, grid),
+                                                  _fp(mv)))
 
     def InitSystemState(self, stream_vel=(0.0, 0.0, 0.0)) -> int:
         n = C.c_uint32()
@@ -167,8 +175,34 @@ class Solver:
         self._c(lib().dsmc_dump_statistics(self._h, path.encode()))
 
     def SampleStatistics(self) -> tuple[np.ndarray, np.ndarray]:
-        """(collision stats {sel, coll, sum c_r, 0}, per-cell sums (C, 5))."""
-        return self.collision_stats(), self.cell_stats()
+        """GPUDSMCSolver::SampleStatistics (include/gpu_dsmc.inl:277-282): the
+        float4 collision totals {sel, coll, sum c_r, 0} and, after one more
+        sample that the samples counter does not include, the per-cell
+        statistics_t records (C, 8) float32 {numInSample, xyzSum[3], velMag, pad}."""
+        coll = np.zeros(4, np.float32)
+        st = np.zeros((self.n_cells, 8), np.float32)
+        self._c(lib().dsmc_sample_statistics(self._h, _fp(coll), _fp(st), self.n_cells))
+        return coll, st
+
+    def sim_state(self) -> dict:
+        """The engine's simulation_state_t counters (simTime, realTime, stepCount, samplesCount)."""
+        st = _capi.SimState()
+        self._c(lib().dsmc_get_sim_state(self._h, C.byref(st)))
+        return {"sim_time": st.sim_time, "real_time": st.real_time, "step_count": st.step_count,
+                "samples_count": st.samples_count}
+
+    def format_statistics(self, sim_time=None, real_time=None, samples=None) -> str:
+        """ProcessAndOutputStatistics text of the current sums (no sampling);
+        counters default to the engine's own."""
+        cur = self.sim_state()
+        st = _capi.SimState(cur["sim_time"] if sim_time is None else sim_time,
+                            cur["real_time"] if real_time is None else real_time, cur["step_count"],
+                            cur["samples_count"] if samples is None else samples)
+        n = C.c_uint64()
+        self._c(lib().dsmc_format_statistics(self._h, C.byref(st), None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        self._c(lib().dsmc_format_statistics(self._h, C.byref(st), buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
 
     # -- extensions
     def set_sampling(self, enabled: bool, reset: bool = True) -> None:

@@ -45,6 +45,13 @@ class CellStats(C.Structure):
     _fields_ = [("num_in_sample", C.c_double), ("xyz_sum", C.c_double * 3), ("vel_mag", C.c_double)]
 
 
+class SimState(C.Structure):
+    """dsmc_sim_state: the counters of simulation_state_t (include/draw_helper.h:40-78)."""
+
+    _fields_ = [("sim_time", C.c_float), ("real_time", C.c_float), ("step_count", C.c_uint32),
+                ("samples_count", C.c_uint32)]
+
+
 assert C.sizeof(Settings) == 496, C.sizeof(Settings)
 
 _lib = None
@@ -77,6 +84,7 @@ def lib() -> C.CDLL:
         "dsmc_init": ([P, pS], I32),
         "dsmc_init_from_config": ([P, C.c_char_p], I32),
         "dsmc_set_geometry": ([P, pF, U32, pU, U32, pF, U32, U32, U32], I32),
+        "dsmc_set_geometry_moved": ([P, pF, U32, pU, U32, pF, U32, U32, U32, pF], I32),
         "dsmc_init_state": ([P, F, F, F, pU], I32),
         "dsmc_init_state_from_fields": ([P, pF, pF, pF, pF, pF, U32, pU], I32),
         "dsmc_step": ([P, F], I32),
@@ -88,6 +96,9 @@ def lib() -> C.CDLL:
         "dsmc_get_collision_stats": ([P, pD], I32),
         "dsmc_get_drag": ([P, pD], I32),
         "dsmc_dump_statistics": ([P, C.c_char_p], I32),
+        "dsmc_sample_statistics": ([P, pF, pF, U32], I32),
+        "dsmc_format_statistics": ([P, C.POINTER(SimState), C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)], I32),
+        "dsmc_get_sim_state": ([P, C.POINTER(SimState)], I32),
         "dsmc_get_counts": ([P, C.POINTER(C.c_uint64), pU, pU], I32),
         "dsmc_synchronize": ([P], I32),
         "dsmc_validate": ([P, pU], I32),
@@ -129,6 +140,7 @@ def lib() -> C.CDLL:
         "dsmc_op_mt_generate":([P, P, P, P, U32, U32, P, P], I32),
         "dsmc_op_cell_ids": ([pS, P, P, P, U32, P, P], I32),
         "dsmc_op_sort": ([P, U32, U32, P, P, P, P], I32),
+        "dsmc_op_sort_work_words": ([U32, U32], C.c_uint64),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -25,6 +25,7 @@ constexpr uint32_t kTileCells = 256;    // most cells per tile (one lane per cel
 constexpr uint32_t kTileThreads = 256;  // threads of the tile kernel
 constexpr uint32_t kLdsCap = 2048;      // particles staged in LDS per tile
 constexpr uint32_t kMtWords = 20;       // 19 window words + 1 spare (zero)
+constexpr uint32_t kRefMtStreams = 65536;  // the reference's MT_RNG_COUNT (MersenneTwister.h:31)
 constexpr int kMtN = 19, kMtM = 9;
 constexpr uint32_t kMtUpper = 0xFFFFFFFEu, kMtLower = 0x1u;
 constexpr int kShift0 = 12, kShiftB = 7, kShiftC = 15, kShift1 = 18;

@@ -16,7 +16,6 @@ void mt_generate(const uint32_t* a, const uint32_t* b, const uint32_t* c, const 
 void cell_ids(const InstDesc& I, const float* x, const float* y, const float* z, uint32_t n, uint32_t* keys,
               hipStream_t st);
 void keys_count(const InstDesc* d, uint32_t par, uint32_t n, hipStream_t st);
-void count_keys(const uint32_t* keys, uint32_t n, uint32_t* cnt, hipStream_t st);
 // Exclusive scan of in[0..n) into out[0..n] (out[n] = total); cursor gets a
 // copy of the offsets when non-null; in is cleared when clear_in; total_out
 // (device) receives the total. scratch: kScanMaxTiles + 1 uint32.
@@ -28,8 +27,12 @@ void scatter(const InstDesc* d, uint32_t par, uint32_t grid, hipStream_t st, uin
 // largest count array of the batch; scratch: n_inst * (kScanMaxTiles + 1).
 void step_scan(const InstDesc* d, uint32_t n_inst, uint32_t which, uint32_t max_n, uint32_t* scratch,
                hipStream_t st);
-void scatter_keys(const uint32_t* keys, uint32_t n, uint32_t* cursor, uint32_t* perm, hipStream_t st);
-void fix_segments(const uint32_t* start, uint32_t n_cells, uint32_t* perm, hipStream_t st);
+// Stable sort operator: cell offsets (n_cells + 1) and the permutation of a
+// stable sort by key (keys >= n_cells dropped, listed last). work:
+// op_sort_work_words(n, n_cells) uint32. No allocation, no memset.
+uint64_t op_sort_work_words(uint32_t n, uint32_t n_cells);
+int op_sort(const uint32_t* keys, uint32_t n, uint32_t n_cells, uint32_t* perm, uint32_t* cell_start,
+            uint32_t* work, hipStream_t st);
 void tile(uint32_t mode, const InstDesc* d, uint32_t par, uint32_t flags, uint32_t n_vwg, hipStream_t st,
           uint32_t n_inst = 1);
 // PHYSICS wall pass over the tiles the tile kernel queued (opar = the

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../common/dsmc_dev.h"
+#include "../common/pairing.h"
 #include "../common/philox.h"
 
 namespace dsmc_dev {
@@ -497,14 +498,14 @@ __device__ __forceinline__ uint32_t move_walls(const InstDesc& I, float& px, flo
 
 // ------------------------------------------------------------------- NTC
 // Isotropic post-collision velocities (dsmc_kernel.cu:1057-1078).
-template <class R>
-__device__ __forceinline__ void scatter_pair(float* vx, float* vy, float* vz, uint32_t i, uint32_t j, float mag,
-                                             R& rng) {
+// cos(theta) = 2 u1 - 1 and azimuth 2 pi u2 from uniforms u1, u2 in (0, 1].
+__device__ __forceinline__ void scatter_uv(float* vx, float* vy, float* vz, uint32_t i, uint32_t j, float mag,
+                                           float u1, float u2) {
   float v1x = vx[i], v1y = vy[i], v1z = vz[i];
   float v2x = vx[j], v2y = vy[j], v2z = vz[j];
-  float b = 2.0f * rng_f(rng) - 1.0f;
+  float b = 2.0f * u1 - 1.0f;
   float a = sqrtf(1.0f - b * b);
-  float c = 6.28318548202514648438f * rng_f(rng);  // 2*DSMC_PI in float
+  float c = 6.28318548202514648438f * u2;  // 2*DSMC_PI in float
   float sn, cs;
   dm_sincosf(c, &sn, &cs);
   float ax = b * mag / 2.0f, ay = a * mag * cs / 2.0f, az = a * mag * sn / 2.0f;
@@ -514,6 +515,14 @@ __device__ __forceinline__ void scatter_pair(float* vx, float* vy, float* vz, ui
   vx[j] = (v1x + v2x) / 2.0f - ax;
   vy[j] = (v1y + v2y) / 2.0f - ay;
   vz[j] = (v1z + v2z) / 2.0f - az;
+}
+
+template <class R>
+__device__ __forceinline__ void scatter_pair(float* vx, float* vy, float* vz, uint32_t i, uint32_t j, float mag,
+                                             R& rng) {
+  const float u1 = rng_f(rng);
+  const float u2 = rng_f(rng);
+  scatter_uv(vx, vy, vz, i, j, mag, u1, u2);
 }
 
 // ComputeBirdCollisionsDev / ComputeBirdCollisionsNoMTDev cell loop
@@ -552,37 +561,6 @@ struct CollAcc {
   double sel, coll, cr;
 };
 
-// NTC with N(N-1)/2 candidate pairs, distinct partners and the VHS
-// cross-section sigma*c_r = vhs_k * c_r^(2-2 omega) (Bird 1994, sec. 4.3).
-// Partners come from the high word of draw * range (no integer divide) and
-// a pair collides when sigma*c_r > R * (sigma*c_r)max (no float divide).
-__device__ __forceinline__ void ntc_physics(const InstDesc& I, float* vx, float* vy, float* vz, uint32_t n,
-                                            float vol, float2& nt, Mt& rng, CollAcc& acc) {
-  if (n < 2u) return;
-  float fpc = (float)(0.5 * (double)n * (double)(n - 1u) * (double)I.fnum * (double)nt.x * (double)I.dt /
-                          (double)vol + (double)nt.y);
-  uint32_t pairs = dm_f2u_sat(fpc);
-  nt.y = fpc - (float)pairs;
-  if (pairs > kMaxPairsPerCell) {
-    pairs = kMaxPairsPerCell;
-    atomicOr(&I.ctr[3], kStatusPairs);
-  }
-  acc.sel += (double)pairs;
-  for (uint32_t k = 0; k < pairs; ++k) {
-    uint32_t i = __umulhi(mt_next(rng), n);
-    uint32_t j = __umulhi(mt_next(rng), n - 1u);
-    if (j >= i) ++j;
-    float dx = vx[i] - vx[j], dy = vy[i] - vy[j], dz = vz[i] - vz[j];
-    float mag = sqrtf(dx * dx + dy * dy + dz * dz);
-    float sc = (I.vhs_exp == 1.0f) ? I.vhs_k * mag : (mag > 0.0f ? I.vhs_k * dm_powf(mag, I.vhs_exp) : 0.0f);
-    if (sc > nt.x) nt.x = sc;
-    if (sc > mt_float(rng) * nt.x) {
-      acc.coll += 1.0;
-      acc.cr += (double)mag;
-      scatter_pair(vx, vy, vz, i, j, mag, rng);
-    }
-  }
-}
 
 // ---------------------------------------------------------- body interior
 // Point inside the body: collision-grid cell filled, or, in a surface cell,

@@ -18,9 +18,11 @@
 //    per cell out of LDS (:1002-1089), moves each particle with walls and
 //    box faces (:1596-1657), writes the next key and samples per cell
 //    (:856-890).
-//  * RNG: stream s = 256 * virtual_workgroup + lane; the tile kernel runs a
-//    fixed grid of n_streams/256 workgroups, each walking tiles vwg,
-//    vwg + n_vwg, ... so cell c always draws from stream c mod n_streams.
+//  * RNG (PHYSICS): stream s = 256 * virtual_workgroup + lane; the tile
+//    kernel runs a fixed grid of n_streams/256 workgroups, each walking a
+//    fixed sequence of tiles, so a cell always draws from the same stream.
+//    REFERENCE mode keeps the reference's layout: cell c collides with
+//    stream c and loading thread t draws from stream t.
 #include <hip/hip_runtime.h>
 
 #include "launch.h"
@@ -151,11 +153,12 @@ __global__ void k_keys_count(const InstDesc* __restrict__ Ip, uint32_t par, uint
   if (i < n) gmem(I.rank)[i] = rk;
 }
 
-// Plain counting of arbitrary keys (sort operator).
-__global__ void k_count_keys(const uint32_t* keys, uint32_t n, uint32_t* cnt) {
+// Plain counting of arbitrary keys (sort operator); keys >= n_cells are
+// dropped.
+__global__ void k_count_keys(const uint32_t* keys, uint32_t n, uint32_t n_cells, uint32_t* cnt) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t key = i < n ? keys[i] : kInvalidKey;
-  count_keys(gmem(cnt), key);
+  count_keys(gmem(cnt), key < n_cells ? key : kInvalidKey);
 }
 
 // ====================================================================== scan
@@ -426,41 +429,92 @@ __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip
   }
 }
 
-// Generic scatter for the sort operator.
-__global__ void k_scatter_keys(const uint32_t* keys, uint32_t n, uint32_t* cursor, uint32_t* perm) {
-  const uint32_t lane = lane_id();
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t key = i < n ? keys[i] : kInvalidKey;
-  uint32_t prev = __shfl_up(key, 1);
-  bool head = (lane == 0) || (key != prev);
-  uint64_t heads = __ballot(head);
-  uint32_t slot = 0;
-  if (head && key != kInvalidKey) {
-    uint64_t above = heads & ~((2ull << lane) - 1ull);
-    if (lane == 63) above = 0;
-    uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
-    slot = atomicAdd(&cursor[key], next - lane);
-  }
-  uint64_t mine = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
-  uint32_t hl = 63u - (uint32_t)__clzll((long long)mine);
-  uint32_t hslot = __shfl(slot, (int)hl);
-  if (key != kInvalidKey) perm[hslot + (lane - hl)] = i;
+// ============================================================ sort operator
+// Stable LSD radix sort of (key, storage index) pairs, 8 bits per pass: the
+// permutation of SortParticlesIndices' thrust::sort_by_key
+// (dsmc_kernel.cu:2033-2051). Keys >= n_cells (dropped particles) sort as
+// n_cells, after every valid key. A pass is a per-block digit histogram
+// (hist[digit * nblocks + block]), an exclusive scan of it, and a scatter in
+// which each block ranks its 4096 keys stably: rounds of 256 positions in
+// order, waves in order within a round, lanes in order within a wave (peers
+// of equal digit found with eight ballots).
+constexpr uint32_t kRsThreads = 256;
+constexpr uint32_t kRsRounds = 16;
+constexpr uint32_t kRsTile = kRsThreads * kRsRounds;  // 4096 keys per block
+
+__device__ __forceinline__ uint32_t rs_key(const uint32_t* keys, uint32_t i, uint32_t n_cells, bool raw) {
+  const uint32_t k = keys[i];
+  return raw ? (k < n_cells ? k : n_cells) : k;
 }
 
-// Stable order inside each cell segment (insertion sort, one lane per cell).
-__global__ void k_fix_segments(const uint32_t* start, uint32_t n_cells, uint32_t* perm) {
-  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n_cells) return;
-  uint32_t a = start[c], b = start[c + 1];
-  for (uint32_t i = a + 1; i < b; ++i) {
-    uint32_t v = perm[i];
-    uint32_t j = i;
-    while (j > a && perm[j - 1] > v) {
-      perm[j] = perm[j - 1];
-      --j;
-    }
-    perm[j] = v;
+__global__ void __launch_bounds__(kRsThreads) k_rs_hist(const uint32_t* keys, uint32_t n, uint32_t n_cells,
+                                                        uint32_t raw, uint32_t shift, uint32_t nblocks,
+                                                        uint32_t* hist) {
+  __shared__ uint32_t s_h[256];
+  const uint32_t tid = threadIdx.x, b = blockIdx.x;
+  s_h[tid] = 0u;
+  __syncthreads();
+  const uint32_t base = b * kRsTile;
+  for (uint32_t r = 0; r < kRsRounds; ++r) {
+    const uint32_t i = base + r * kRsThreads + tid;
+    if (i < n) atomicAdd(&s_h[(rs_key(keys, i, n_cells, raw != 0u) >> shift) & 255u], 1u);
   }
+  __syncthreads();
+  hist[(size_t)tid * nblocks + b] = s_h[tid];
+}
+
+__global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const uint32_t* keys, const uint32_t* vals, uint32_t n,
+                                                           uint32_t n_cells, uint32_t raw, uint32_t shift,
+                                                           uint32_t nblocks, const uint32_t* offs,
+                                                           uint32_t* keys_out, uint32_t* vals_out) {
+  __shared__ uint32_t s_run[256];              // this block's keys of each digit in earlier rounds
+  __shared__ uint32_t s_wc[kRsThreads / 64][256];  // keys of each digit per wave in this round
+  const uint32_t tid = threadIdx.x, b = blockIdx.x, lane = lane_id(), w = tid / 64u;
+  s_run[tid] = 0u;
+#pragma unroll
+  for (uint32_t k = 0; k < kRsThreads / 64; ++k) s_wc[k][tid] = 0u;
+  const uint32_t dbase = offs[(size_t)tid * nblocks + b];  // block's first slot of digit tid
+  __shared__ uint32_t s_base[256];
+  s_base[tid] = dbase;
+  __syncthreads();
+  const uint64_t below = (1ull << lane) - 1ull;
+  const uint32_t base = b * kRsTile;
+  for (uint32_t r = 0; r < kRsRounds; ++r) {
+    const uint32_t i = base + r * kRsThreads + tid;
+    const bool valid = i < n;
+    const uint32_t k = valid ? rs_key(keys, i, n_cells, raw != 0u) : 0u;
+    const uint32_t v = valid ? (vals ? vals[i] : i) : 0u;
+    const uint32_t d = (k >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (uint32_t bit = 0; bit < 8; ++bit) {
+      const uint64_t m = __ballot((d >> bit) & 1u);
+      peers &= ((d >> bit) & 1u) ? m : ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & below);
+    if (valid && rank == 0u) s_wc[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t off = s_run[d] + rank;
+      for (uint32_t k2 = 0; k2 < w; ++k2) off += s_wc[k2][d];
+      const uint32_t dst = s_base[d] + off;
+      keys_out[dst] = k;
+      vals_out[dst] = v;
+    }
+    __syncthreads();
+    uint32_t t = 0u;
+#pragma unroll
+    for (uint32_t k2 = 0; k2 < kRsThreads / 64; ++k2) {
+      t += s_wc[k2][tid];
+      s_wc[k2][tid] = 0u;
+    }
+    s_run[tid] += t;
+    __syncthreads();
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
 // ====================================================================== tile
@@ -474,6 +528,121 @@ constexpr uint32_t kMoveBatch = 6;  // rounds of 256 positions whose loads are i
 // stores them to LDS)
 constexpr uint32_t kPermPrefetch = 4;
 
+// LDS of the PHYSICS NTC rounds.
+struct NtcLds {
+  uint32_t* off;   // [kTileCells] each cell's first item of the round
+  uint32_t* nmax;  // [kTileCells] (sigma c_r)max at the round's start (float bits)
+  uint32_t* rmax;  // [kTileCells] largest sigma c_r seen so far (float bits)
+  uint8_t* icell;  // [kTileThreads] cell of each of the round's first kTileThreads items
+  uint32_t* red;   // [8] wave totals of the scan, then the waves' largest round counts
+};
+
+// PHYSICS NTC of one tile in rounds of disjoint pairs (common/pairing.h).
+// Lane l < nc computes cell l's candidate count M; round r holds
+// min(floor(n / 2), M - r floor(n / 2)) pairs of each cell, and the pairs of
+// all cells of a round are dealt to every thread of the workgroup (item t of
+// the round: its cell from the scanned pair counts). The pairs of a round
+// share no particle, so each is tested against the round's starting
+// (sigma c_r)max and collided independently; the maximum then takes the
+// round's largest sigma c_r. Draws: Philox4x32-10 under key
+// (seed, kNtcKey1) at counter (cell, step, round, item).
+__device__ __forceinline__ void tile_ntc_rounds(const InstDesc& I, uint32_t c0, uint32_t nc, uint32_t p0,
+                                                const uint32_t* s_start, float* vx, float* vy, float* vz,
+                                                float2& nt, float frac, uint32_t stepno, const NtcLds& L,
+                                                CollAcc& acc) {
+  using namespace dsmc_pairing;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / 64u;
+  uint32_t n = 0u, M = 0u;
+  if (tid < nc) {
+    n = s_start[tid + 1] - s_start[tid];
+    if (n >= 2u) {
+      const float vol = I.cell_volume * (I.fluid_frac ? fmaxf(frac, kMinFluidFrac) : 1.0f);
+      const float fpc = (float)(0.5 * (double)n * (double)(n - 1u) * (double)I.fnum * (double)nt.x *
+                                    (double)I.dt / (double)vol +
+                                (double)nt.y);
+      M = dm_f2u_sat(fpc);
+      nt.y = fpc - (float)M;
+      if (M > kMaxPairsPerCell) {
+        M = kMaxPairsPerCell;
+        atomicOr(&I.ctr[3], kStatusPairs);
+      }
+      acc.sel += (double)M;
+    }
+    L.nmax[tid] = __float_as_uint(nt.x);
+    L.rmax[tid] = __float_as_uint(nt.x);
+  }
+  // rounds of the tile: the most over its cells
+  uint32_t R = rounds_of(n, M);
+  for (int o = 32; o > 0; o >>= 1) R = max(R, (uint32_t)__shfl_xor((int)R, o));
+  if (lane == 0) L.red[4 + wid] = R;
+  __syncthreads();
+  uint32_t rounds = 0u;
+#pragma unroll
+  for (uint32_t w = 0; w < kTileThreads / 64; ++w) rounds = max(rounds, L.red[4 + w]);
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t m = round_pairs(n, M, r);
+    // exclusive scan of the round's pair counts over the cells
+    uint32_t x = m;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) L.red[wid] = x;
+    __syncthreads();
+    uint32_t base = 0u, total = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < kTileThreads / 64; ++w) {
+      const uint32_t t = L.red[w];
+      base += w < wid ? t : 0u;
+      total += t;
+    }
+    const uint32_t off = base + x - m;
+    if (tid < nc) {
+      L.off[tid] = off;
+      for (uint32_t q = off; q < off + m && q < kTileThreads; ++q) L.icell[q] = (uint8_t)tid;
+    }
+    __syncthreads();
+    for (uint32_t it = tid; it < total; it += kTileThreads) {
+      uint32_t c;
+      if (it < kTileThreads) {
+        c = L.icell[it];
+      } else {  // the last cell whose first item is at or before it
+        uint32_t lo = 0u, hi = nc - 1u;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1u) >> 1;
+          if (L.off[mid] <= it) lo = mid;
+          else hi = mid - 1u;
+        }
+        c = lo;
+      }
+      const uint32_t oc = L.off[c];
+      const uint32_t mc = (c + 1u < nc ? L.off[c + 1u] : total) - oc;
+      const uint32_t k = it - oc;
+      const uint32_t a = s_start[c] - p0, nn = s_start[c + 1u] - s_start[c];
+      const dsmc_rng::U32x4 rk = dsmc_rng::philox4x32_10({{c0 + c, stepno, r, kRoundItem}}, I.wall_key[0], kNtcKey1);
+      const dsmc_rng::U32x4 d = dsmc_rng::philox4x32_10({{c0 + c, stepno, r, k}}, I.wall_key[0], kNtcKey1);
+      uint32_t i, j;
+      pair_members(nn, mc, k, rk, i, j);
+      i += a;
+      j += a;
+      const float dx = vx[i] - vx[j], dy = vy[i] - vy[j], dz = vz[i] - vz[j];
+      const float mag = sqrtf(dx * dx + dy * dy + dz * dz);
+      const float sc =
+          (I.vhs_exp == 1.0f) ? I.vhs_k * mag : (mag > 0.0f ? I.vhs_k * dm_powf(mag, I.vhs_exp) : 0.0f);
+      atomicMax(&L.rmax[c], __float_as_uint(sc));
+      const float max0 = __uint_as_float(L.nmax[c]);
+      if (sc > dsmc_rng::px_unit(d.v[0]) * max0) {
+        acc.coll += 1.0;
+        acc.cr += (double)mag;
+        scatter_uv(vx, vy, vz, i, j, mag, dsmc_rng::px_unit(d.v[1]), dsmc_rng::px_unit(d.v[2]));
+      }
+    }
+    __syncthreads();
+    if (tid < nc) L.nmax[tid] = L.rmax[tid];
+  }
+  if (tid < nc) nt.x = __uint_as_float(L.nmax[tid]);
+}
+
 template <uint32_t MODE, bool LDS>
 __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint32_t flags, uint32_t tile,
                                           uint32_t c0, uint32_t nc, uint32_t p0, uint32_t np,
@@ -481,7 +650,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
                                           uint32_t* sn_start, uint8_t* sn_nearc, uint32_t (&pf)[kPermPrefetch],
                                           uint32_t* s_raw,
                                           uint32_t* s_perm, uint8_t* s_owner, float* s_vx, float* s_vy,
-                                          float* s_vz, Mt& rng, CollAcc& cacc) {
+                                          float* s_vz, uint32_t stepno, const NtcLds& ntl, CollAcc& cacc) {
   const uint32_t tid = threadIdx.x;
   const uint32_t opar = par ^ 1u;
   const gptr<const float> ipx = gmem<const float>(I.px[par]);
@@ -585,14 +754,14 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
     if (tid == 0) nx_end = gmem(I.cell_start)[c0n + ncn];
     if (MODE == kModePhysics) nx_near = (I.near_cell && tid < ncn) ? gmem(I.near_cell)[c0n + tid] : (uint8_t)255;
   }
-  // (4) NTC collisions, one lane per cell
-  if (tid < nc) {
-    uint32_t c = c0 + tid;
-    uint32_t a = s_start[tid] - p0, b = s_start[tid + 1] - p0;
-    uint32_t n = b - a;
-    if (n >= 2u) {
-      float2 nt = nt_pf;
-      if (MODE == kModeReference) {
+  // (4) NTC collisions: REFERENCE one lane per cell (the reference's loop),
+  // PHYSICS in rounds of disjoint pairs over the whole workgroup
+  if (MODE == kModeReference) {
+    if (tid < nc) {
+      const uint32_t c = c0 + tid;
+      const uint32_t a = s_start[tid] - p0, n = s_start[tid + 1] - s_start[tid];
+      if (n >= 2u) {
+        float2 nt = nt_pf;
         float4 st = st_pf;
         if (I.taus) {
           uint4 q = gload(gmem<const uint4>(reinterpret_cast<const uint4*>(I.taus)) + c);
@@ -600,15 +769,21 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
           ntc_reference(I, vx + a, vy + a, vz + a, n, nt, st, t);
           gstore(gmem(reinterpret_cast<uint4*>(I.taus)) + c, make_uint4(t.x, t.y, t.z, t.w));
         } else {
-          ntc_reference(I, vx + a, vy + a, vz + a, n, nt, st, rng);
+          // ComputeBirdCollisionsDev draws from the cell's own stream
+          // (ds_mtState[cellid], dsmc_kernel.cu:1030-1031)
+          Mt cr;
+          mt_load(I, c, cr);
+          ntc_reference(I, vx + a, vy + a, vz + a, n, nt, st, cr);
+          mt_store(I, c, cr);
         }
         gstore(gmem(I.cstat) + c, st);
-      } else {
-        float vol = I.cell_volume * (I.fluid_frac ? fmaxf(frac_pf, kMinFluidFrac) : 1.0f);
-        ntc_physics(I, vx + a, vy + a, vz + a, n, vol, nt, rng, cacc);
+        gstore(gmem(I.ntc) + c, nt);
       }
-      gstore(gmem(I.ntc) + c, nt);
     }
+  } else {
+    float2 nt = nt_pf;
+    tile_ntc_rounds(I, c0, nc, p0, s_start, vx, vy, vz, nt, frac_pf, stepno, ntl, cacc);
+    if (tid < nc && s_start[tid + 1] - s_start[tid] >= 2u) gstore(gmem(I.ntc) + c0 + tid, nt);
   }
   if (tile_n != kInvalidKey) {
     sn_start[tid] = nx_start;
@@ -761,19 +936,24 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
   __shared__ float s_vx[kLdsCap];
   __shared__ float s_vy[kLdsCap];
   __shared__ double s_red[kTileThreads / 64][kAccSlots];
+  // PHYSICS NTC rounds (the item-cell table shares s_red, used only at the end)
+  __shared__ uint32_t s_ntc_off[kTileCells];
+  __shared__ uint32_t s_ntc_nmax[kTileCells];
+  __shared__ uint32_t s_ntc_rmax[kTileCells];
+  __shared__ uint32_t s_ntc_red[8];
+  static_assert(sizeof(s_red) >= kTileThreads, "item-cell table must fit in s_red");
+  const NtcLds ntl = {s_ntc_off, s_ntc_nmax, s_ntc_rmax, reinterpret_cast<uint8_t*>(&s_red[0][0]), s_ntc_red};
   // The raw permutation is dead once the rank sort has run (phase 2), so
   // the z velocities reuse its space: 35 KB of LDS, four workgroups per CU.
   float* const s_vz = reinterpret_cast<float*>(s_raw);
 
   const uint32_t tid = threadIdx.x;
   const uint32_t vw = blockIdx.x;
-  const uint32_t stream = vw * kTileThreads + tid;
+  const uint32_t stepno = gmem(I.ctr)[4];  // this step's number (counter of the PHYSICS draws)
   if (vw == 0 && tid == 0) I.ctr[0] = I.ctr[1];
   TileWalk w = tile_walk_begin(I.n_vwg, vw);
   uint32_t j = tile_walk_pos(w);
   if (j >= I.ntiles) return;  // no tile for this workgroup: its streams and sums stay as they are
-  Mt rng;
-  mt_load(I, stream, rng);
   CollAcc cacc = {0.0, 0.0, 0.0};
   uint32_t tile = kInvalidKey;
   if (j < I.ntiles) {
@@ -806,14 +986,13 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
     const uint32_t np = s_start[b][nc] - p0;
     if (np <= kLdsCap)
       tile_body<MODE, true>(I, par, flags, tile, c0, nc, p0, np, s_start[b], s_nearc[b], tile_n, s_start[b ^ 1u],
-                            s_nearc[b ^ 1u], pf, s_raw, s_perm, s_owner, s_vx, s_vy, s_vz, rng, cacc);
+                            s_nearc[b ^ 1u], pf, s_raw, s_perm, s_owner, s_vx, s_vy, s_vz, stepno, ntl, cacc);
     else
       tile_body<MODE, false>(I, par, flags, tile, c0, nc, p0, np, s_start[b], s_nearc[b], tile_n, s_start[b ^ 1u],
-                             s_nearc[b ^ 1u], pf, s_raw, s_perm, s_owner, s_vx, s_vy, s_vz, rng, cacc);
+                             s_nearc[b ^ 1u], pf, s_raw, s_perm, s_owner, s_vx, s_vy, s_vz, stepno, ntl, cacc);
     __syncthreads();
     tile = tile_n;
   }
-  mt_store(I, stream, rng);
 
   // per-workgroup accumulators (deterministic: fixed reduction order)
   double v[kAccSlots] = {cacc.sel, cacc.coll, cacc.cr, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1341,11 +1520,6 @@ void keys_count(const InstDesc* d, uint32_t par, uint32_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_keys_count, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, par, n);
 }
 
-void count_keys(const uint32_t* keys, uint32_t n, uint32_t* cnt, hipStream_t st) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_count_keys, dim3(blocks_for(n, 256)), dim3(256), 0, st, keys, n, cnt);
-}
-
 int scan(uint32_t* in, uint32_t n, uint32_t* out, uint32_t* cursor, bool clear_in, uint32_t* scratch,
          uint32_t* total_out, hipStream_t st) {
   uint32_t tiles = blocks_for(n > 0 ? n : 1u, kScanTile);
@@ -1370,14 +1544,47 @@ void step_scan(const InstDesc* d, uint32_t n_inst, uint32_t which, uint32_t max_
   hipLaunchKernelGGL(k_bscan_apply, dim3(tiles, n_inst), dim3(kScanThreads), 0, st, d, which, scratch);
 }
 
-void scatter_keys(const uint32_t* keys, uint32_t n, uint32_t* cursor, uint32_t* perm, hipStream_t st) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_scatter_keys, dim3(blocks_for(n, 256)), dim3(256), 0, st, keys, n, cursor, perm);
+uint64_t op_sort_work_words(uint32_t n, uint32_t n_cells) {
+  const uint64_t nb = blocks_for(n > 0 ? n : 1u, kRsTile);
+  return (uint64_t)n_cells + 2ull * (256ull * nb + 1ull) + (kScanMaxTiles + 1ull) + 3ull * n;
 }
 
-void fix_segments(const uint32_t* start, uint32_t n_cells, uint32_t* perm, hipStream_t st) {
-  if (!n_cells) return;
-  hipLaunchKernelGGL(k_fix_segments, dim3(blocks_for(n_cells, 256)), dim3(256), 0, st, start, n_cells, perm);
+int op_sort(const uint32_t* keys, uint32_t n, uint32_t n_cells, uint32_t* perm, uint32_t* cell_start,
+            uint32_t* work, hipStream_t st) {
+  const uint32_t nb = blocks_for(n > 0 ? n : 1u, kRsTile);
+  if ((uint64_t)n_cells > (uint64_t)kScanMaxTiles * kScanTile || 256ull * nb > (uint64_t)kScanMaxTiles * kScanTile)
+    return -1;
+  uint32_t* cnt = work;
+  uint32_t* hist = cnt + n_cells;
+  uint32_t* offs = hist + (256u * nb + 1u);
+  uint32_t* scr = offs + (256u * nb + 1u);
+  uint32_t* ka = scr + (kScanMaxTiles + 1u);
+  uint32_t* kb = ka + n;
+  uint32_t* vt = kb + n;
+  // cell offsets: counts of the valid keys and their exclusive scan
+  if (n_cells) hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(blocks_for(n_cells, 256), 4096u)), dim3(256),
+                                  0, st, cnt, n_cells, 0u);
+  if (n) hipLaunchKernelGGL(k_count_keys, dim3(blocks_for(n, 256)), dim3(256), 0, st, keys, n, n_cells, cnt);
+  if (scan(cnt, n_cells, cell_start, nullptr, true, scr, nullptr, st)) return -1;
+  if (!n) return 0;
+  // radix passes over the bits of n_cells (the largest mapped key)
+  uint32_t bits = 1;
+  while (bits < 32 && (n_cells >> bits) != 0u) ++bits;
+  const uint32_t passes = (bits + 7u) / 8u;
+  const uint32_t* kin = keys;
+  const uint32_t* vin = nullptr;
+  for (uint32_t p = 0; p < passes; ++p) {
+    const uint32_t raw = p == 0 ? 1u : 0u;
+    uint32_t* kout = (p & 1u) ? kb : ka;
+    uint32_t* vout = ((passes - 1u - p) & 1u) ? vt : perm;  // the last pass writes perm
+    hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(kRsThreads), 0, st, kin, n, n_cells, raw, 8u * p, nb, hist);
+    if (scan(hist, 256u * nb, offs, nullptr, false, scr, nullptr, st)) return -1;
+    hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(kRsThreads), 0, st, kin, vin, n, n_cells, raw, 8u * p, nb,
+                       offs, kout, vout);
+    kin = kout;
+    vin = vout;
+  }
+  return 0;
 }
 
 void walls(const InstDesc* d, uint32_t opar, uint32_t flags, uint32_t grid, hipStream_t st, uint32_t n_inst) {

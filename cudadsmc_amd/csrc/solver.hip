@@ -15,6 +15,7 @@
 #include <fstream>
 #include <map>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -54,6 +55,12 @@ struct dsmc_solver {
   uint32_t par = 0;
   uint64_t steps = 0;
   uint32_t samples = 0;
+  // simulation_state_t::simTime: FP32 sum of dt over the steps, as
+  // advanceSimTime accumulates it (include/draw_helper.h:71)
+  float sim_time = 0.0f;
+  void advance_time(uint64_t n) {
+    for (uint64_t k = 0; k < n; ++k) sim_time += h.dt;
+  }
   bool sampling = false;
   uint64_t sample_steps0 = 0;
   double acc_snap[kAccSlots] = {0};
@@ -306,7 +313,19 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
     ns = std::min(wg, 4096u) * kTileThreads;
   }
   ns = round_up(ns, kTileThreads);
-  if (!phys) ns = std::max(ns, round_up(std::max(C, (uint32_t)(s.grid_dim[0] * s.grid_dim[1]) * h->ppc), kTileThreads));
+  if (!phys) {
+    // REFERENCE mode keeps the reference's stream layout: loading thread
+    // X*Y*i + y*X + x (dsmc_kernel.cu:633) and, with -mt, cell c in the
+    // collisions (:1030-1031) index the stream table directly. The reference
+    // allocates MT_RNG_COUNT = 65536 streams (MersenneTwister.h:31) and
+    // overruns them beyond that, so larger cases have no reference result.
+    const uint64_t load_threads = (uint64_t)s.grid_dim[0] * s.grid_dim[1] * h->ppc;
+    if (load_threads > kRefMtStreams || (s.mt && C > kRefMtStreams))
+      return h->fail(DSMC_ERR_INVALID,
+                     "REFERENCE mode: %llu loading threads and %u cells exceed the reference's %u MT streams",
+                     (unsigned long long)load_threads, C, kRefMtStreams);
+    ns = std::max(ns, round_up(std::max(C, (uint32_t)load_threads), kTileThreads));
+  }
   I.cap = cap;
   I.n_streams = ns;
   I.n_vwg = ns / kTileThreads;
@@ -443,6 +462,11 @@ int dsmc_init_from_config(dsmc_solver* h, const char* path) {
 
 int dsmc_set_geometry(dsmc_solver* h, const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
                       const float* normals, uint32_t gx, uint32_t gy, uint32_t gz) {
+  return dsmc_set_geometry_moved(h, verts, nv, tris, nt, normals, gx, gy, gz, nullptr);
+}
+
+int dsmc_set_geometry_moved(dsmc_solver* h, const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                            const float* normals, uint32_t gx, uint32_t gy, uint32_t gz, const float move[3]) {
   if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
   if (!h->inited) return h->fail(DSMC_ERR_STATE, "set_geometry before init");
   if (h->have_state) return h->fail(DSMC_ERR_STATE, "set_geometry after the particles were loaded");
@@ -458,6 +482,15 @@ int dsmc_set_geometry(dsmc_solver* h, const float* verts, uint32_t nv, const uin
   if (h->physics()) build_cgrid_physics(m, gx, gy, gz, h->cg);
   else build_cgrid_reference(m, gx, gy, gz, h->cg);
   CGrid& g = h->cg;
+  if (move) {
+    // Model::MoveModel after CreateGrid (src/model.cpp:709-731): vertices and
+    // grid bounds shift, the cell size and the triangle lists stay
+    for (size_t i = 0; i < m.v.size(); ++i) m.v[i] += move[i % 3];
+    for (int k = 0; k < 3; ++k) {
+      g.mn[k] += move[k];
+      g.mx[k] += move[k];
+    }
+  }
   h->free_geom();
   InstDesc& I = h->h;
   uint32_t* flag;
@@ -756,6 +789,7 @@ int dsmc_run(dsmc_solver* h, uint32_t steps, float dt) {
     uint32_t f2 = step_flags(h, h->steps + 2, &s2);
     CHECK_RC(run_graph_pair(h, f1, f2));
     h->steps += 2;
+    h->advance_time(2);
     h->samples += (s1 ? 1u : 0u) + (s2 ? 1u : 0u);
     done += 2;
   }
@@ -907,39 +941,35 @@ int dsmc_get_drag(dsmc_solver* h, double out[4]) {
   return DSMC_OK;
 }
 
-int dsmc_dump_statistics(dsmc_solver* h, const char* path) {
-  if (!h || !path) return set_global_error(DSMC_ERR_INVALID, "null argument");
-  if (!h->have_state) return h->fail(DSMC_ERR_STATE, "no state");
-  // DumpStatistics -> SampleStatistics samples once more without counting it
-  // (include/gpu_dsmc.inl:277-282, kernels/dsmc_base.cu:165-174)
+namespace {
+
+// ProcessAndOutputStatistics (kernels/dsmc_base.cu:104-163) from the current
+// per-cell sums and the given simulation-state counters.
+int format_statistics(dsmc_solver* h, const dsmc_sim_state& st, std::string& text) {
   double coll[4];
   CHECK_RC(dsmc_get_collision_stats(h, coll));
-  dsmc_launch::sample_cells(h->d_desc, h->par, h->h.mode, h->h.ncells, h->stream);
-  HIPC(hipGetLastError());
   InstDesc& I = h->h;
   std::vector<dsmc_cell_stats> cs(I.ncells);
   CHECK_RC(dsmc_get_cell_stats(h, cs.data(), I.ncells));
-  std::ofstream out(path);
-  if (!out) return h->fail(DSMC_ERR_IO, "cannot write %s", path);
-  // ProcessAndOutputStatistics (kernels/dsmc_base.cu:104-163)
+  std::ostringstream out;
+  // GetStatistics hands over a float4 (dsmc_kernel.cu:840-854)
   const float sel = (float)coll[0], col = (float)coll[1], cr = (float)coll[2];
   out << "Selections: " << sel << "\nCollisions: " << col << "\nMean separation: " << cr / col
       << "\nCollision ratio: " << col / sel << "\n";
-  const float sim_time = (float)((double)h->steps * I.dt);
-  out << "Simulation time: " << sim_time << " Running time: " << 0 << "\n";
+  out << "Simulation time: " << st.sim_time << " Running time: " << st.real_time << "\n";
   out << "Cell N  ||  SampNum  ||  Dens  ||          Velocity          ||  Temperature\n";
   double tot = 0, denn = 0, avg_denn = 0;
-  const float cell_volume = I.cell_volume;
+  const float cell_volume = h->s.cell_size[0] * h->s.cell_size[1] * h->s.cell_size[2];
   const float kb = kBoltzRef;
   for (uint32_t i = 0; i < I.ncells; ++i) {
-    const dsmc_cell_stats& s = cs[i];
-    float a = h->s.fnum / (cell_volume * (float)h->samples);
-    denn = a * (float)s.num_in_sample;
+    const dsmc_cell_stats& c = cs[i];
+    float a = h->s.fnum / (cell_volume * (float)st.samples_count);
+    const float n = (float)c.num_in_sample;
+    denn = a * n;
     avg_denn += denn;
-    float n = (float)s.num_in_sample;
-    float vx = (float)s.xyz_sum[0] / n, vy = (float)s.xyz_sum[1] / n, vz = (float)s.xyz_sum[2] / n;
+    float vx = (float)c.xyz_sum[0] / n, vy = (float)c.xyz_sum[1] / n, vz = (float)c.xyz_sum[2] / n;
     float uu = vx * vx + vy * vy + vz * vz;
-    float tt = h->s.molmass * ((float)s.vel_mag / n - uu) / (3.0f * kb);
+    float tt = h->s.molmass * ((float)c.vel_mag / n - uu) / (3.0f * kb);
     tot += tt;
     out << i << "  ||  " << (uint32_t)n << "  ||  " << denn << "  ||  " << vx << "," << vy << "," << vz << "  ||  "
         << tt << "\n";
@@ -947,15 +977,90 @@ int dsmc_dump_statistics(dsmc_solver* h, const char* path) {
   tot /= I.ncells;
   avg_denn /= I.ncells;
   const float gv = (h->s.bmax[0] - h->s.bmin[0]) * (h->s.bmax[1] - h->s.bmin[1]) * (h->s.bmax[2] - h->s.bmin[2]);
+  // the reference names these dens2 / diam2 but forms n*d twice; the product
+  // n^2 d^2 is the hard-sphere collision-rate factor
   const double dens2 = (double)h->s.density * h->s.diameter, diam2 = (double)h->s.diameter * h->s.density;
   const double molmas = h->s.molmass;
   const float Tref = h->s.Tref;
-  float tColl = (float)(2.0f * sim_time * dens2 * gv * kPiRef * diam2 * std::pow(tot / Tref, 0.25) *
+  float tColl = (float)(2.0f * st.sim_time * dens2 * gv * kPiRef * diam2 * std::pow(tot / Tref, 0.25) *
                         sqrtf((float)(kb * Tref / (kPiRef * molmas))) / h->s.fnum);
   out << "\n\nExpected collisions: " << tColl << " Ratio: " << tColl / col << "\n";
   out << "Average temperature: " << tot << " Reference temperature is: " << h->s.temp
       << "\nAverage density: " << denn << " Reference density: " << h->s.density << "\n\n" << std::endl;
+  text = out.str();
   return DSMC_OK;
+}
+
+dsmc_sim_state engine_state(const dsmc_solver* h) {
+  dsmc_sim_state st{};
+  st.sim_time = h->sim_time;
+  st.real_time = 0.0f;  // the benchmark path never advances realTime (kernels/dsmc.cu:270 is interactive)
+  st.step_count = (uint32_t)h->steps;
+  st.samples_count = h->samples;
+  return st;
+}
+
+}  // namespace
+
+int dsmc_get_sim_state(dsmc_solver* h, dsmc_sim_state* out) {
+  if (!h || !out) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  *out = engine_state(h);
+  return DSMC_OK;
+}
+
+int dsmc_sample_statistics(dsmc_solver* h, float coll_out[4], float* stats, uint32_t n_cells) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->have_state) return h->fail(DSMC_ERR_STATE, "no state");
+  InstDesc& I = h->h;
+  if (stats && n_cells < I.ncells) return h->fail(DSMC_ERR_INVALID, "need %u cells", I.ncells);
+  // SampleStatistics (include/gpu_dsmc.inl:277-282): GetStatistics, then one
+  // more SampleCellStatistics that the samples counter does not include
+  double coll[4];
+  CHECK_RC(dsmc_get_collision_stats(h, coll));
+  if (coll_out)
+    for (int k = 0; k < 4; ++k) coll_out[k] = (float)coll[k];
+  dsmc_launch::sample_cells(h->d_desc, h->par, h->h.mode, h->h.ncells, h->stream);
+  HIPC(hipGetLastError());
+  if (stats) {
+    std::vector<dsmc_cell_stats> cs(I.ncells);
+    CHECK_RC(dsmc_get_cell_stats(h, cs.data(), I.ncells));
+    for (uint32_t c = 0; c < I.ncells; ++c) {
+      float* r = stats + (size_t)8 * c;  // statistics_t: n, xyzSum[3], velMag, pad (8 floats)
+      r[0] = (float)cs[c].num_in_sample;
+      r[1] = (float)cs[c].xyz_sum[0];
+      r[2] = (float)cs[c].xyz_sum[1];
+      r[3] = (float)cs[c].xyz_sum[2];
+      r[4] = (float)cs[c].vel_mag;
+      r[5] = r[6] = r[7] = 0.0f;
+    }
+  }
+  return DSMC_OK;
+}
+
+int dsmc_format_statistics(dsmc_solver* h, const dsmc_sim_state* st, char* buf, uint64_t cap, uint64_t* len) {
+  if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
+  if (!h->have_state) return h->fail(DSMC_ERR_STATE, "no state");
+  std::string text;
+  CHECK_RC(format_statistics(h, st ? *st : engine_state(h), text));
+  if (len) *len = text.size();
+  if (!buf) return DSMC_OK;
+  if (cap < text.size() + 1) return h->fail(DSMC_ERR_CAPACITY, "statistics need %zu bytes", text.size() + 1);
+  std::memcpy(buf, text.c_str(), text.size() + 1);
+  return DSMC_OK;
+}
+
+int dsmc_dump_statistics(dsmc_solver* h, const char* path) {
+  if (!h || !path) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  if (!h->have_state) return h->fail(DSMC_ERR_STATE, "no state");
+  // DumpStatistics = SampleStatistics + ProcessAndOutputStatistics
+  // (kernels/dsmc_base.cu:165-174)
+  CHECK_RC(dsmc_sample_statistics(h, nullptr, nullptr, 0));
+  std::string text;
+  CHECK_RC(format_statistics(h, engine_state(h), text));
+  std::ofstream out(path);
+  if (!out) return h->fail(DSMC_ERR_IO, "cannot write %s", path);
+  out << text;
+  return out ? DSMC_OK : h->fail(DSMC_ERR_IO, "cannot write %s", path);
 }
 
 int dsmc_get_counts(dsmc_solver* h, uint64_t* steps, uint32_t* n_particles, uint32_t* samples) {
@@ -1307,24 +1412,18 @@ int dsmc_op_cell_ids(const dsmc_settings* s, const float* x, const float* y, con
   return DSMC_OK;
 }
 
+uint64_t dsmc_op_sort_work_words(uint32_t n, uint32_t n_cells) { return dsmc_launch::op_sort_work_words(n, n_cells); }
+
 int dsmc_op_sort(const uint32_t* keys, uint32_t n, uint32_t n_cells, uint32_t* perm, uint32_t* cell_start,
                  uint32_t* work, void* stream) {
-  if (!keys || !perm || !cell_start || !work) return set_global_error(DSMC_ERR_INVALID, "null argument");
-  hipStream_t st = (hipStream_t)stream;
-  uint32_t* cnt = work;
-  uint32_t* cur = work + n_cells;
-  uint32_t* scr = work + 2 * (size_t)n_cells;
-  hipError_t e = hipMemsetAsync(cnt, 0, (size_t)n_cells * 4, st);
-  if (e != hipSuccess) return set_global_error(DSMC_ERR_DEVICE, hipGetErrorString(e));
-  dsmc_launch::count_keys(keys, n, cnt, st);
-  if (dsmc_launch::scan(cnt, n_cells, cell_start, cur, true, scr, nullptr, st))
-    return set_global_error(DSMC_ERR_CAPACITY, "too many cells");
-  dsmc_launch::scatter_keys(keys, n, cur, perm, st);
-  dsmc_launch::fix_segments(cell_start, n_cells, perm, st);
-  e = hipGetLastError();
+  if ((n && (!keys || !perm)) || !cell_start || !work) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  if (dsmc_launch::op_sort(keys, n, n_cells, perm, cell_start, work, (hipStream_t)stream))
+    return set_global_error(DSMC_ERR_CAPACITY, "too many keys or cells for the scan");
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_global_error(DSMC_ERR_DEVICE, hipGetErrorString(e));
   return DSMC_OK;
 }
+
 
 }  // extern "C"
 
@@ -1393,6 +1492,7 @@ int step_once(dsmc_solver* h) {
   CHECK_RC(enqueue_step(h, h->par, flags, ev));
   HIPC(hipGetLastError());
   h->par ^= 1u;
+  h->advance_time(stepno - h->steps);
   h->steps = stepno;
   if (sample) h->samples += 1;
   return DSMC_OK;
@@ -1604,6 +1704,7 @@ int dsmc_batch_run(dsmc_batch* b, uint32_t steps) {
   BHIP(hipGetLastError());
   for (auto* h : b->hs) {
     h->par = par;
+    h->advance_time(stepno - h->steps);
     h->steps = stepno;
     if (!h->physics() || h->sampling) h->samples += samples;
   }

@@ -106,6 +106,16 @@ typedef struct dsmc_cell_stats {
   double vel_mag;
 } dsmc_cell_stats;
 
+/* The counters of simulation_state_t (include/draw_helper.h:40-78) that the
+ * statistics output reads: simulated time (FP32 sum of dt per step, as
+ * advanceSimTime), wall time, steps and samples. */
+typedef struct dsmc_sim_state {
+  float sim_time;
+  float real_time;
+  uint32_t step_count;
+  uint32_t samples_count;
+} dsmc_sim_state;
+
 typedef struct dsmc_solver dsmc_solver;
 
 /* ---------------------------------------------------------------- settings */
@@ -141,6 +151,13 @@ int dsmc_init_from_config(dsmc_solver* h, const char* config_path);
 int dsmc_set_geometry(dsmc_solver* h, const float* vertices, uint32_t n_vertices,
                       const uint32_t* triangles, uint32_t n_triangles,
                       const float* face_normals, uint32_t gx, uint32_t gy, uint32_t gz);
+/* dsmc_set_geometry followed by Model::MoveModel (src/model.cpp:709-731),
+ * the order of the reference's driver (kernels/dsmc.cu:403-405): the grid is
+ * built over the given vertices, then vertices and grid bounds shift by
+ * move[3] while the cell size and triangle lists stay. */
+int dsmc_set_geometry_moved(dsmc_solver* h, const float* vertices, uint32_t n_vertices,
+                            const uint32_t* triangles, uint32_t n_triangles, const float* face_normals,
+                            uint32_t gx, uint32_t gy, uint32_t gz, const float move[3]);
 /* GPUDSMCSolver::InitSystemState (include/gpu_dsmc.inl:69-92). */
 int dsmc_init_state(dsmc_solver* h, float ux, float uy, float uz, uint32_t* n_out);
 /* Warm start: load local Maxwellians from per-cell fields (n [m^-3], u,v,w
@@ -170,9 +187,24 @@ int dsmc_get_collision_stats(dsmc_solver* h, double out[4]);
 /* Body force {Fx, Fy, Fz} [N] and heat flux [W] averaged over the sampling
  * window (PHYSICS mode; zero otherwise). */
 int dsmc_get_drag(dsmc_solver* h, double out[4]);
-/* DSMCSolver::DumpStatistics (kernels/dsmc_base.cu:165-174): writes the
- * reference's statistics.txt table. */
+/* DSMCSolver::DumpStatistics (kernels/dsmc_base.cu:165-174): one more,
+ * uncounted sample, then the reference's statistics.txt table with the
+ * engine's counters (real time 0, as the reference's benchmark path leaves
+ * it). */
 int dsmc_dump_statistics(dsmc_solver* h, const char* path);
+/* GPUDSMCSolver::SampleStatistics (include/gpu_dsmc.inl:277-282): the
+ * collision totals as the float4 of GetStatistics, one more sample that the
+ * samples counter does not include, and per cell the statistics_t record
+ * (include/statistics.h:31-37) as 8 floats {numInSample, xyzSum[3], velMag,
+ * pad[3]} (stats may be NULL). */
+int dsmc_sample_statistics(dsmc_solver* h, float coll_out[4], float* stats, uint32_t n_cells);
+/* ProcessAndOutputStatistics (kernels/dsmc_base.cu:104-163) of the current
+ * sums as text, with the counters of st (NULL: the engine's own, see
+ * dsmc_get_sim_state). *len receives the text length; buf (cap bytes, NUL
+ * included) may be NULL to query it. Does not sample. */
+int dsmc_format_statistics(dsmc_solver* h, const dsmc_sim_state* st, char* buf, uint64_t cap, uint64_t* len);
+/* The engine's counters in simulation_state_t form (real_time 0). */
+int dsmc_get_sim_state(dsmc_solver* h, dsmc_sim_state* out);
 /* Counters: steps taken, live particles, samples taken. */
 int dsmc_get_counts(dsmc_solver* h, uint64_t* steps, uint32_t* n_particles, uint32_t* samples);
 /* Wait for all queued work of the handle. */
@@ -305,12 +337,15 @@ int dsmc_op_mt_generate(const uint32_t* d_a, const uint32_t* d_b, const uint32_t
 int dsmc_op_cell_ids(const dsmc_settings* s, const float* d_x, const float* d_y, const float* d_z,
                      uint32_t n, uint32_t* d_keys, void* stream);
 /* SortParticlesIndices + FindCellBoundaries bounds (dsmc_kernel.cu:1734-1808,
- * 2033-2051): stable sort of keys (< n_cells, or 0xFFFFFFFF = dropped) into
- * a permutation (d_perm[n]) and cell offsets (d_cell_start[n_cells + 1]).
- * d_work must hold 2*n_cells + 4097 uint32. */
+ * 2033-2051): stable sort of keys into a permutation (d_perm[n]: indices of
+ * the keys < n_cells in key order, ties in index order, then the indices of
+ * the dropped keys >= n_cells) and cell offsets (d_cell_start[n_cells + 1]).
+ * An LSD radix sort, 8 bits per pass over the bits of n_cells, with the
+ * workspace d_work of dsmc_op_sort_work_words(n, n_cells) uint32 (the
+ * reference's thrust call allocated its temporary storage itself). */
+uint64_t dsmc_op_sort_work_words(uint32_t n, uint32_t n_cells);
 int dsmc_op_sort(const uint32_t* d_keys, uint32_t n, uint32_t n_cells, uint32_t* d_perm,
                  uint32_t* d_cell_start, uint32_t* d_work, void* stream);
-
 /* ------------------------------------ field operator (warm-start loading) */
 /* The reference has no counterpart: these kernels train and evaluate the 3-D
  * convolutional operator that predicts steady fields (n, u, v, w, T) on the

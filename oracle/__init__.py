@@ -62,6 +62,8 @@ def lib() -> C.CDLL:
         "oracle_get_state": ([P, pF, pF, pU, U32], I32),
         "oracle_load": ([P, F, F, F], I32),
         "oracle_step": ([P, I32], I32),
+        "oracle_ntc_cell": ([P, U32, U32, pF, pF, pF, U32, F, pF, pD], I32),
+        "oracle_pair_perm": ([U32, pU, U32], U32),
         "oracle_get_perm": ([P, pU, U32], I32),
         "oracle_get_cell_start": ([P, pU, U32], I32),
         "oracle_get_hits": ([P, pU, U32], I32),
@@ -69,6 +71,7 @@ def lib() -> C.CDLL:
         "oracle_get_cstat": ([P, pF, U32], I32),
         "oracle_get_acc": ([P, pD, U32], I32),
         "oracle_threads": ([], I32),
+        "oracle_set_threads": ([I32], None),
         "oracle_set_taus": ([P, pU, U32], I32),
         "oracle_get_taus": ([P, pU, U32], I32),
         "oracle_taus_draws": ([pU, I32, U32, pU], None),

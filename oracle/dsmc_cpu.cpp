@@ -19,20 +19,21 @@
 // and the PHYSICS extensions (VHS NTC, nearest-hit walls by brute force over
 // all triangles, diffuse re-emission, freestream faces and injection).
 //
-// The engine's RNG stream assignment is part of the algorithm. Tiles are
-// visited in a locality order (sorted by (z / B, y / B, z mod B, y mod B, x)
-// of their first cell, B the largest power of two with B^2 x <= 128
-// tile_cells). When G = n_streams / threads is a multiple of 8, the order is
-// cut into patches of 128 positions, patch p belongs to "XCD" p mod 8, and
-// workgroup w takes elements w / 8, w / 8 + G / 8, ... of the concatenated
-// patches of XCD w mod 8 (element e is position
-// ((e / 128) * 8 + w mod 8) * 128 + e mod 128) until a position passes the
-// tile count; else workgroup w takes positions w, w + G, ...
-// Cell lane l of a tile draws from stream w * threads + l and
-// the move of tile position q from stream w * threads + q mod threads;
-// loading draws from stream (item mod n_streams); injection and the wall
-// re-emission draw counter-based Philox4x32-10. The oracle follows that
-// assignment with plain loops.
+// The oracle visits the tiles in the engine's locality order (sorted by
+// (z / B, y / B, z mod B, y mod B, x) of their first cell, B the largest
+// power of two with B^2 x <= 128 tile_cells; with G = n_streams / threads a
+// multiple of 8, patches of 128 positions dealt to 8 "XCDs" and taken by
+// workgroup w as elements w / 8, w / 8 + G / 8, ... of its XCD's patches,
+// element e being position ((e / 128) * 8 + w mod 8) * 128 + e mod 128;
+// else positions w, w + G, ...). No draw depends on that order; it only
+// groups the per-workgroup double sums of the collision counters.
+// PHYSICS collisions, injection and the wall re-emission draw counter-based
+// Philox4x32-10 (keyed by cell/slot, step and item, so no draw depends on
+// the schedule); PHYSICS loading draws from MT stream (cell mod n_streams).
+// REFERENCE mode keeps the reference's own assignment: cell c collides with
+// stream c (ds_mtState[cellid], dsmc_kernel.cu:1030-1031) and loading
+// thread X*Y*i + y*X + x draws from that stream (:633). The oracle follows
+// that assignment with plain loops.
 #include "dsmc_cpu.h"
 
 #include <algorithm>
@@ -45,6 +46,7 @@
 #endif
 
 #include "../cudadsmc_amd/csrc/common/detmath.h"
+#include "../cudadsmc_amd/csrc/common/pairing.h"
 #include "../cudadsmc_amd/csrc/common/philox.h"
 
 namespace {
@@ -497,10 +499,17 @@ struct oracle_sim {
   // ------------------------------------------------------------- NTC
   template <class R>
   static void scatter(float* ux, float* uy, float* uz, uint32_t i, uint32_t j, float mag, R& r) {
+    const float u1 = r.uniform();
+    const float u2 = r.uniform();
+    scatter_uv(ux, uy, uz, i, j, mag, u1, u2);
+  }
+  // isotropic post-collision velocities from two uniforms in (0, 1]:
+  // cos(theta) = 2 u1 - 1, azimuth 2 pi u2 (dsmc_kernel.cu:1057-1078)
+  static void scatter_uv(float* ux, float* uy, float* uz, uint32_t i, uint32_t j, float mag, float u1, float u2) {
     float v1x = ux[i], v1y = uy[i], v1z = uz[i], v2x = ux[j], v2y = uy[j], v2z = uz[j];
-    float b = 2.0f * r.uniform() - 1.0f;
+    float b = 2.0f * u1 - 1.0f;
     float a = std::sqrt(1 - b * b);
-    float c = 2 * PI_REF * r.uniform();
+    float c = 2 * PI_REF * u2;
     float sn, cs;
     dm_sincosf(c, &sn, &cs);
     float ax = b * mag / 2.0f, ay = a * mag * cs / 2.0f, az = a * mag * sn / 2.0f;
@@ -535,7 +544,15 @@ struct oracle_sim {
     }
   }
 
-  void ntc_physics(float* ux, float* uy, float* uz, uint32_t n, float vol, float* nt, MtBlock& r, double* a) const {
+  // PHYSICS NTC: M = floor(n (n - 1) / 2 F (sigma c_r)max dt / V + rem)
+  // candidates in rounds of disjoint pairs (pairing.h). Every pair of a round
+  // is tested against the round's starting (sigma c_r)max, then the maximum
+  // takes the largest sigma c_r of the round; VHS cross-section
+  // sigma c_r = vhs_k c_r^(2 - 2 omega). Draws: Philox4x32-10 under
+  // (seed, kNtcKey1) at (cell, step, round, item).
+  void ntc_physics(float* ux, float* uy, float* uz, uint32_t n, float vol, float* nt, uint32_t cell,
+                   double* a) const {
+    using namespace dsmc_pairing;
     if (n < 2) return;
     float fpc = (float)(0.5 * (double)n * (double)(n - 1) * (double)fnum * (double)nt[0] * (double)dt / (double)vol +
                         (double)nt[1]);
@@ -543,20 +560,27 @@ struct oracle_sim {
     nt[1] = fpc - (float)pairs;
     pairs = std::min(pairs, kMaxPairs);
     a[0] += pairs;
-    for (uint32_t k = 0; k < pairs; ++k) {
-      // partner = high word of draw * range; collide when sc > R * max
-      uint32_t i = (uint32_t)(((uint64_t)r.next() * n) >> 32);
-      uint32_t j = (uint32_t)(((uint64_t)r.next() * (n - 1)) >> 32);
-      if (j >= i) ++j;
-      float dx = ux[i] - ux[j], dy = uy[i] - uy[j], dz = uz[i] - uz[j];
-      float mag = std::sqrt(dx * dx + dy * dy + dz * dz);
-      float sc = vhs_exp == 1.0f ? vhs_k * mag : (mag > 0.0f ? vhs_k * dm_powf(mag, vhs_exp) : 0.0f);
-      if (sc > nt[0]) nt[0] = sc;
-      if (sc > r.uniform() * nt[0]) {
-        a[1] += 1.0;
-        a[2] += mag;
-        scatter(ux, uy, uz, i, j, mag, r);
+    const uint32_t rounds = rounds_of(n, pairs);
+    for (uint32_t r = 0; r < rounds; ++r) {
+      const uint32_t m = round_pairs(n, pairs, r);
+      const dsmc_rng::U32x4 rk = dsmc_rng::philox4x32_10({{cell, stepno, r, kRoundItem}}, (uint32_t)s.seed, kNtcKey1);
+      const float max0 = nt[0];
+      float rmax = max0;
+      for (uint32_t k = 0; k < m; ++k) {
+        uint32_t i, j;
+        pair_members(n, m, k, rk, i, j);
+        const dsmc_rng::U32x4 d = dsmc_rng::philox4x32_10({{cell, stepno, r, k}}, (uint32_t)s.seed, kNtcKey1);
+        float dx = ux[i] - ux[j], dy = uy[i] - uy[j], dz = uz[i] - uz[j];
+        float mag = std::sqrt(dx * dx + dy * dy + dz * dz);
+        float sc = vhs_exp == 1.0f ? vhs_k * mag : (mag > 0.0f ? vhs_k * dm_powf(mag, vhs_exp) : 0.0f);
+        if (sc > rmax) rmax = sc;
+        if (sc > dsmc_rng::px_unit(d.v[0]) * max0) {
+          a[1] += 1.0;
+          a[2] += mag;
+          scatter_uv(ux, uy, uz, i, j, mag, dsmc_rng::px_unit(d.v[1]), dsmc_rng::px_unit(d.v[2]));
+        }
       }
+      nt[0] = rmax;
     }
   }
 
@@ -674,29 +698,27 @@ struct oracle_sim {
         const uint32_t tile = order[j];
         uint32_t c0 = tile * TC, nc = std::min(TC, C - c0);
         uint32_t p0 = cell_start[c0], p1 = cell_start[c0 + nc];
-        // NTC, one stream per cell lane
+        // NTC: PHYSICS one stream per cell lane, REFERENCE the cell's own stream
         for (uint32_t l = 0; l < nc; ++l) {
           uint32_t c = c0 + l;
           uint32_t a0 = cell_start[c], b0 = cell_start[c + 1];
-          MtBlock& r = rng[(size_t)vw * TT + l];
           if (phys) {
             float vol = cell_volume * (frac.empty() ? 1.0f : std::max(frac[c], 1.0f / 64.0f));
-            ntc_physics(&nvx[a0], &nvy[a0], &nvz[a0], b0 - a0, vol, &ntc[2 * (size_t)c], r, a);
+            ntc_physics(&nvx[a0], &nvy[a0], &nvz[a0], b0 - a0, vol, &ntc[2 * (size_t)c], c, a);
           } else if (!taus.empty()) {
             ntc_reference(&nvx[a0], &nvy[a0], &nvz[a0], (int)(b0 - a0), &ntc[2 * (size_t)c], &cstat[4 * (size_t)c],
                           taus[c]);
           } else {
-            ntc_reference(&nvx[a0], &nvy[a0], &nvz[a0], (int)(b0 - a0), &ntc[2 * (size_t)c], &cstat[4 * (size_t)c], r);
+            ntc_reference(&nvx[a0], &nvy[a0], &nvz[a0], (int)(b0 - a0), &ntc[2 * (size_t)c], &cstat[4 * (size_t)c],
+                          rng[c]);
           }
         }
-        // move, one stream per position modulo the thread count
+        // move (no draws: wall re-emission is counter-based)
         uint32_t cq = c0;
         for (uint32_t q = p0; q < p1; ++q) {
           while (q >= cell_start[cq + 1]) ++cq;
-          MtBlock& r = rng[(size_t)vw * TT + (q - p0) % TT];
           uint32_t nh = 0, k;
           if (phys) {
-            (void)r;
             k = near_body(cq, nvx[q], nvy[q], nvz[q]) ? DEFER
                                                       : move_free(nx[q], ny[q], nz[q], nvx[q], nvy[q], nvz[q]);
           } else {
@@ -982,7 +1004,7 @@ void oracle_stable_sort(const uint32_t* keys, uint32_t n, uint32_t n_cells, uint
   std::vector<uint32_t> idx;
   idx.reserve(n);
   for (uint32_t i = 0; i < n; ++i)
-    if (keys[i] != INV) idx.push_back(i);
+    if (keys[i] < n_cells) idx.push_back(i);  // keys >= n_cells are dropped
   std::stable_sort(idx.begin(), idx.end(), [&](uint32_t p, uint32_t q) { return keys[p] < keys[q]; });
   std::copy(idx.begin(), idx.end(), perm);
   for (uint32_t c = 0; c <= n_cells; ++c) {
@@ -1242,6 +1264,23 @@ int oracle_load(oracle_sim* o, float ux, float uy, float uz) {
   return 0;
 }
 
+int oracle_ntc_cell(oracle_sim* o, uint32_t cell, uint32_t stepno, float* vx, float* vy, float* vz, uint32_t n,
+                    float vol, float* nt, double* acc3) {
+  if (!o || !o->phys) return -1;
+  const uint32_t saved = o->stepno;
+  o->stepno = stepno;
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  o->ntc_physics(vx, vy, vz, n, vol, nt, cell, a);
+  o->stepno = saved;
+  for (int k = 0; k < 3; ++k) acc3[k] = a[k];
+  return 0;
+}
+
+uint32_t oracle_pair_perm(uint32_t n, const uint32_t keys[4], uint32_t x) {
+  dsmc_rng::U32x4 k = {{keys[0], keys[1], keys[2], keys[3]}};
+  return dsmc_pairing::pp_apply(dsmc_pairing::pp_make(n, k), x);
+}
+
 int oracle_step(oracle_sim* o, int sample) {
   if (!o) return -1;
   o->step(sample != 0);
@@ -1293,6 +1332,14 @@ int oracle_threads(void) {
   return omp_get_max_threads();
 #else
   return 1;
+#endif
+}
+
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
 #endif
 }
 

@@ -76,6 +76,13 @@ int oracle_get_state(oracle_sim* o, float* pos, float* vel, uint32_t* keys, uint
 int oracle_load(oracle_sim* o, float ux, float uy, float uz);
 /* One step; sample != 0 accumulates the per-cell sums. */
 int oracle_step(oracle_sim* o, int sample);
+/* PHYSICS NTC of one cell (n particles, cell-local arrays, fluid volume
+ * vol, nt = {(sigma c_r)max, remainder}) at step `stepno`; acc3 receives
+ * {candidates, collisions, sum of c_r of the collisions}. */
+int oracle_ntc_cell(oracle_sim* o, uint32_t cell, uint32_t stepno, float* vx, float* vy, float* vz, uint32_t n,
+                    float vol, float* nt, double* acc3);
+/* The keyed permutation of the PHYSICS pairing: pi(x) on [0, n). */
+uint32_t oracle_pair_perm(uint32_t n, const uint32_t keys[4], uint32_t x);
 int oracle_get_perm(oracle_sim* o, uint32_t* out, uint32_t n);
 int oracle_get_cell_start(oracle_sim* o, uint32_t* out, uint32_t n);
 int oracle_get_hits(oracle_sim* o, uint32_t* out, uint32_t n);
@@ -87,6 +94,8 @@ int oracle_get_cstat(oracle_sim* o, float* out, uint32_t n_cells);
 int oracle_get_acc(oracle_sim* o, double* out, uint32_t n);
 /* Number of threads used by oracle_step (OpenMP). */
 int oracle_threads(void);
+/* OpenMP threads of the following calls (timing a one-thread baseline). */
+void oracle_set_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -99,14 +99,15 @@ def _gpu_sort(keys: np.ndarray, n_cells: int):
     tk = dev(keys.view(np.int32)) if n else torch.zeros(1, dtype=torch.int32, device="cuda")
     perm = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
     cs = torch.zeros(n_cells + 1, dtype=torch.int32, device="cuda")
-    work = torch.zeros(3 * n_cells + 8192, dtype=torch.int32, device="cuda")
+    work = torch.zeros(int(_capi.lib().dsmc_op_sort_work_words(n, n_cells)), dtype=torch.int32, device="cuda")
     _capi.check(_capi.lib().dsmc_op_sort(ptr(tk), n, n_cells, ptr(perm), ptr(cs), ptr(work), None))
     torch.cuda.synchronize()
     cs_ = cs.cpu().numpy().view(np.uint32)
     return perm.cpu().numpy().view(np.uint32)[: cs_[-1]], cs_
 
 
-@pytest.mark.parametrize("case", ["random", "nearly_sorted", "one_cell", "invalid", "empty", "many_cells"])
+@pytest.mark.parametrize("case", ["random", "nearly_sorted", "one_cell", "one_cell_large", "invalid", "out_of_range",
+                                  "empty", "single", "many_cells", "one_pass_cells"])
 def test_gpu_stable_sort(case):
     rng = np.random.default_rng(11)
     C_ = 65536
@@ -118,19 +119,29 @@ def test_gpu_stable_sort(case):
         keys[mv] = np.clip(keys[mv].astype(np.int64) + rng.choice([-1, 1, -256, 256], mv.sum()), 0, C_ - 1)
     elif case == "one_cell":
         keys = np.full(20000, 123, np.uint32)
+    elif case == "one_cell_large":  # a single segment of 2M keys (linear work per radix pass)
+        keys = np.full(2_000_000, 65535, np.uint32)
     elif case == "invalid":
         keys = rng.integers(0, C_, 100000).astype(np.uint32)
         keys[rng.random(len(keys)) < 0.2] = 0xFFFFFFFF
+    elif case == "out_of_range":  # any key >= n_cells is dropped, not only 0xFFFFFFFF
+        keys = rng.integers(0, 2 * C_, 100000).astype(np.uint32)
     elif case == "empty":
         keys = np.zeros(0, np.uint32)
+    elif case == "single":
+        keys = np.array([7], np.uint32)
+    elif case == "one_pass_cells":  # n_cells < 256: one radix pass
+        C_ = 200
+        keys = rng.integers(0, 210, 300000).astype(np.uint32)
     else:
         C_ = 4 * 1024 * 1024
         keys = rng.integers(0, C_, 2_000_000).astype(np.uint32)
+    note(f"sort case {case}: {len(keys)} keys, {C_} cells")
     want_p, want_c = oracle.stable_sort(keys, C_)
     got_p, got_c = _gpu_sort(keys, C_)
     assert np.array_equal(got_c, want_c)
     assert np.array_equal(got_p, want_p)
-    valid = keys != 0xFFFFFFFF
+    valid = keys < C_
     assert np.array_equal(want_p, np.argsort(np.where(valid, keys.astype(np.int64), 2**32),
                                              kind="stable")[: valid.sum()])
 
