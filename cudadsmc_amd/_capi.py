@@ -141,6 +141,7 @@ def lib() -> C.CDLL:
         "dsmc_op_cell_ids": ([pS, P, P, P, U32, P, P], I32),
         "dsmc_op_sort": ([P, U32, U32, P, P, P, P], I32),
         "dsmc_op_sort_work_words": ([U32, U32], C.c_uint64),
+        "dsmc_op_sort_static": ([P, U32, U32, P, P, P], I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

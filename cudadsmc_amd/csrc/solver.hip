@@ -15,6 +15,7 @@
 #include <fstream>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -1424,6 +1425,65 @@ int dsmc_op_sort(const uint32_t* keys, uint32_t n, uint32_t n_cells, uint32_t* p
   return DSMC_OK;
 }
 
+// ---------------------------------------------------- sort without allocation
+// dsmc_op_sort with its workspace in a file-scope device array instead of a
+// caller buffer, for callers that cannot allocate (code under HIP graph
+// capture, serving loops). Limits: n and n_cells up to 2^20; the array
+// (op_sort_work_words(2^20, 2^20) words, 17.3 MB) is held on every GPU the
+// library runs on until the process ends. Every call on a GPU uses that
+// GPU's copy: calls that overlap on one GPU race.
+namespace {
+constexpr uint32_t kStaticSortMax = 1u << 20;
+constexpr uint64_t kStaticSortBlocks = (kStaticSortMax + 4095u) / 4096u;
+constexpr uint64_t kStaticSortWords =
+    kStaticSortMax + 2ull * (256ull * kStaticSortBlocks + 1ull) + (dsmc_launch::kScanMaxTiles + 1ull) +
+    3ull * kStaticSortMax;
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) uint32_t scratch[kStaticSortWords];
+
+namespace {
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (host code must not pass the symbol itself to a kernel).
+int scratch_on_current_device(uint32_t** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_global_error(DSMC_ERR_DEVICE, hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return set_global_error(DSMC_ERR_DEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return set_global_error(DSMC_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed");
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return DSMC_OK;
+}
+}  // namespace
+
+int dsmc_op_sort_static(const uint32_t* keys, uint32_t n, uint32_t n_cells, uint32_t* perm, uint32_t* cell_start,
+                        void* stream) {
+  if ((n && (!keys || !perm)) || !cell_start) return set_global_error(DSMC_ERR_INVALID, "null argument");
+  if (n > kStaticSortMax || n_cells > kStaticSortMax)
+    return set_global_error(DSMC_ERR_CAPACITY, "dsmc_op_sort_static takes at most " + std::to_string(kStaticSortMax) +
+                                                   " keys and cells (got " + std::to_string(n) + " keys, " +
+                                                   std::to_string(n_cells) + " cells)");
+  if (dsmc_launch::op_sort_work_words(n, n_cells) > kStaticSortWords)
+    return set_global_error(DSMC_ERR_CAPACITY, "workspace exceeds the static array");
+  uint32_t* work = nullptr;
+  int rc = scratch_on_current_device(&work);
+  if (rc != DSMC_OK) return rc;
+  if (dsmc_launch::op_sort(keys, n, n_cells, perm, cell_start, work, (hipStream_t)stream))
+    return set_global_error(DSMC_ERR_CAPACITY, "too many keys or cells for the scan");
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_global_error(DSMC_ERR_DEVICE, hipGetErrorString(e));
+  return DSMC_OK;
+}
 
 }  // extern "C"
 

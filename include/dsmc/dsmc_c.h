@@ -346,6 +346,16 @@ int dsmc_op_cell_ids(const dsmc_settings* s, const float* d_x, const float* d_y,
 uint64_t dsmc_op_sort_work_words(uint32_t n, uint32_t n_cells);
 int dsmc_op_sort(const uint32_t* d_keys, uint32_t n, uint32_t n_cells, uint32_t* d_perm,
                  uint32_t* d_cell_start, uint32_t* d_work, void* stream);
+/* dsmc_op_sort without the d_work argument. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops). Allocates nothing: the workspace is a
+ * file-scope device array of 17.3 MB, held on each GPU the library runs on
+ * until the process ends. n and n_cells at most 2^20 (DSMC_ERR_CAPACITY
+ * beyond). All calls on one GPU share its array, so calls that overlap on one
+ * GPU (several streams or host threads) race. */
+int dsmc_op_sort_static(const uint32_t* d_keys, uint32_t n, uint32_t n_cells, uint32_t* d_perm,
+                        uint32_t* d_cell_start, void* stream);
+
 /* ------------------------------------ field operator (warm-start loading) */
 /* The reference has no counterpart: these kernels train and evaluate the 3-D
  * convolutional operator that predicts steady fields (n, u, v, w, T) on the
