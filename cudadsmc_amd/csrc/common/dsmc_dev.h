@@ -49,6 +49,9 @@ constexpr float kMinFluidFrac = 1.0f / 64.0f;
 // a corrupted state (non-finite (sigma c_r)max); the count is clamped and
 // status bit kStatusPairs is raised instead of spinning for ~2^32 pairs.
 constexpr uint32_t kMaxPairsPerCell = 1u << 20;
+// Fixed-point unit of the summed relative speeds of the collision counters:
+// each collision adds round(c_r * 2^16).
+constexpr double kCrScale = 65536.0;
 // Bits of the sticky status word ctr[3].
 constexpr uint32_t kStatusCapacity = 1u, kStatusPairs = 2u;
 
@@ -66,6 +69,7 @@ struct InstDesc {
   uint32_t ncells;
   uint32_t tile_cells;    // cells per tile: a power of two in [32, kTileCells]
   uint32_t ntiles;
+  uint32_t tile_dyn;      // tile kernel takes tiles from atomic queues (else a fixed sequence)
   float cell_volume;
   // ---- gas and NTC
   float mass;
@@ -124,7 +128,7 @@ struct InstDesc {
   float near_unit;      // smallest cell size / 64
   // ---- scalars and per-workgroup accumulators
   uint32_t* ctr;        // [0] stored, [1] sorted, [2] injected particles, [3] status bits, [4] step number,
-                        // [5] deferred particles of the step
+                        // [5] deferred particles of the step, [8..15] the tile kernel's tile queues
   const uint32_t* tile_order;  // tiles in locality order [ntiles] (null: identity)
   unsigned long long* defer_bits;  // PHYSICS with a body: bit s set = slot s deferred [cap / 64 + 2]
   uint32_t* defer_list; // PHYSICS with a body: deferred slots of this step, compacted [cap]
@@ -136,7 +140,7 @@ struct InstDesc {
   double wall_unit[2];  // value of one fixed-point unit: m v_ref / 2^36 (momentum), m v_ref^2 / 2^34 (energy)
   double* wall_acc;     // PHYSICS with a body: wall {Fx, Fy, Fz, heat} sums over the steps [4]
   uint32_t wall_key[2]; // Philox key of the wall re-emission draws
-  double* wg_acc;       // [n_vwg][kAccSlots]
+  long long* wg_acc;    // [n_vwg][kAccSlots] collision counters per workgroup (integers, see CollAcc)
   uint32_t* hits;       // per sorted particle wall hits (debug, may be null)
   // ---- RNG
   uint32_t* taus;       // REFERENCE with mt = 0: Hybrid-Taus state per cell, 4 words (else null)
@@ -222,7 +226,20 @@ __host__ __device__ inline uint32_t tile_walk_pos(const TileWalk& w) {
   return ((w.e / kTilePatch) * 8u + w.xcd) * kTilePatch + w.e % kTilePatch;
 }
 
-// Per virtual-workgroup accumulator slots (doubles).
+// Tile queues of the tile kernel: workgroups take tiles dynamically, one
+// atomic increment per tile. With a workgroup count that is a multiple of 8
+// there is one queue per XCD (workgroup w uses queue w mod 8); element e of
+// queue q is position ((e / kTilePatch) * 8 + q) * kTilePatch + e mod
+// kTilePatch of the locality order, i.e. the queue runs through that XCD's
+// patches in order, so neighbouring tiles run together under one L2 and
+// every XCD gets the same share of the domain. Else one queue over all
+// positions. No result depends on which workgroup takes which tile.
+__host__ __device__ inline uint32_t tile_queue_pos(uint32_t e, uint32_t q, bool per_xcd) {
+  return per_xcd ? ((e / kTilePatch) * 8u + q) * kTilePatch + e % kTilePatch : e;
+}
+constexpr uint32_t kTileQueueCtr = 8;  // ctr[8 + q]: next element of queue q
+
+// Per virtual-workgroup accumulator slots.
 enum AccSlot {
   kAccSel = 0, kAccColl, kAccCr, kAccFx, kAccFy, kAccFz, kAccHeat, kAccHits, kAccSlots
 };

@@ -557,8 +557,13 @@ __device__ __forceinline__ void ntc_reference(const InstDesc& I, float* vx, floa
   }
 }
 
+// Collision counters of a lane: candidates, collisions and the sum of the
+// collisions' relative speeds in units of 2^-16 m/s (kCrScale). Integer sums
+// do not depend on the order of their terms, so the totals are the same for
+// any assignment of tiles to workgroups.
 struct CollAcc {
-  double sel, coll, cr;
+  unsigned long long sel, coll;
+  long long cr;
 };
 
 

@@ -346,6 +346,7 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_tiles(const InstDesc* __
       I.ctr[1] = total;
       I.ctr[4] += 1u;  // step number (counter of the wall draws)
       I.ctr[5] = 0u;   // deferred particles of the step
+      for (uint32_t q = 0; q < 8u; ++q) I.ctr[kTileQueueCtr + q] = 0u;  // the tile kernel's queues
     }
   }
 }
@@ -566,19 +567,16 @@ __device__ __forceinline__ void tile_ntc_rounds(const InstDesc& I, uint32_t c0, 
         M = kMaxPairsPerCell;
         atomicOr(&I.ctr[3], kStatusPairs);
       }
-      acc.sel += (double)M;
+      acc.sel += M;
     }
     L.nmax[tid] = __float_as_uint(nt.x);
     L.rmax[tid] = __float_as_uint(nt.x);
   }
-  // rounds of the tile: the most over its cells
+  // rounds of the tile (the most over its cells), found with the first
+  // round's scan
   uint32_t R = rounds_of(n, M);
   for (int o = 32; o > 0; o >>= 1) R = max(R, (uint32_t)__shfl_xor((int)R, o));
-  if (lane == 0) L.red[4 + wid] = R;
-  __syncthreads();
-  uint32_t rounds = 0u;
-#pragma unroll
-  for (uint32_t w = 0; w < kTileThreads / 64; ++w) rounds = max(rounds, L.red[4 + w]);
+  uint32_t rounds = 1u;
   for (uint32_t r = 0; r < rounds; ++r) {
     const uint32_t m = round_pairs(n, M, r);
     // exclusive scan of the round's pair counts over the cells
@@ -588,6 +586,7 @@ __device__ __forceinline__ void tile_ntc_rounds(const InstDesc& I, uint32_t c0, 
       if (lane >= (uint32_t)o) x += y;
     }
     if (lane == 63) L.red[wid] = x;
+    if (r == 0u && lane == 0) L.red[4 + wid] = R;
     __syncthreads();
     uint32_t base = 0u, total = 0u;
 #pragma unroll
@@ -595,7 +594,9 @@ __device__ __forceinline__ void tile_ntc_rounds(const InstDesc& I, uint32_t c0, 
       const uint32_t t = L.red[w];
       base += w < wid ? t : 0u;
       total += t;
+      if (r == 0u) rounds = max(rounds, L.red[4 + w]);
     }
+    if (total == 0u) break;  // uniform: round 0 without pairs means a tile without candidates
     const uint32_t off = base + x - m;
     if (tid < nc) {
       L.off[tid] = off;
@@ -632,21 +633,23 @@ __device__ __forceinline__ void tile_ntc_rounds(const InstDesc& I, uint32_t c0, 
       atomicMax(&L.rmax[c], __float_as_uint(sc));
       const float max0 = __uint_as_float(L.nmax[c]);
       if (sc > dsmc_rng::px_unit(d.v[0]) * max0) {
-        acc.coll += 1.0;
-        acc.cr += (double)mag;
+        acc.coll += 1u;
+        acc.cr += __double2ll_rn((double)mag * kCrScale);
         scatter_uv(vx, vy, vz, i, j, mag, dsmc_rng::px_unit(d.v[1]), dsmc_rng::px_unit(d.v[2]));
       }
     }
     __syncthreads();
     if (tid < nc) L.nmax[tid] = L.rmax[tid];
   }
-  if (tid < nc) nt.x = __uint_as_float(L.nmax[tid]);
+  if (tid < nc) nt.x = __uint_as_float(L.rmax[tid]);
 }
 
 template <uint32_t MODE, bool LDS>
 __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint32_t flags, uint32_t tile,
                                           uint32_t c0, uint32_t nc, uint32_t p0, uint32_t np,
-                                          const uint32_t* s_start, const uint8_t* s_nearc, uint32_t tile_n,
+                                          const uint32_t* s_start, const uint8_t* s_nearc, uint32_t& tile_n,
+                                          gptr<uint32_t> queue, uint32_t qx, bool per_xcd, uint32_t e_static,
+                                          uint32_t* s_pop,
                                           uint32_t* sn_start, uint8_t* sn_nearc, uint32_t (&pf)[kPermPrefetch],
                                           uint32_t* s_raw,
                                           uint32_t* s_perm, uint8_t* s_owner, float* s_vx, float* s_vy,
@@ -673,6 +676,12 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
   float* vy = LDS ? s_vy : I.vy[opar] + p0;
   float* vz = LDS ? s_vz : I.vz[opar] + p0;
 
+  // the workgroup's next tile: taken from its queue now (dynamic
+  // schedule; the atomic's latency is covered by phases 1 and 2), or the
+  // next element of its fixed sequence
+  const bool dyn = I.tile_dyn != 0u;
+  uint32_t popped = 0u;
+  if (dyn && tid == 0) popped = gatomic_add(queue, 1u);
   // (1) raw permutation of the tile and the owning cell of each position
   if (LDS) {
     if (MODE == kModePhysics) {
@@ -706,7 +715,12 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
     for (; j < b; ++j) r0 += raw[j] < v ? 1u : 0u;
     srt[a + r0 + r1 + r2 + r3] = v;
   }
+  if (dyn && tid == 0) *s_pop = popped;
   __syncthreads();
+  {
+    const uint32_t jn = tile_queue_pos(dyn ? *s_pop : e_static, qx, per_xcd);
+    tile_n = jn < I.ntiles ? (I.tile_order ? gmem(I.tile_order)[jn] : jn) : kInvalidKey;
+  }
   // the collision lane's cell state, loaded now so that its latency overlaps
   // the velocity gathers
   float2 nt_pf = make_float2(0.0f, 0.0f);
@@ -951,12 +965,25 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
   const uint32_t vw = blockIdx.x;
   const uint32_t stepno = gmem(I.ctr)[4];  // this step's number (counter of the PHYSICS draws)
   if (vw == 0 && tid == 0) I.ctr[0] = I.ctr[1];
-  TileWalk w = tile_walk_begin(I.n_vwg, vw);
-  uint32_t j = tile_walk_pos(w);
-  if (j >= I.ntiles) return;  // no tile for this workgroup: its streams and sums stay as they are
-  CollAcc cacc = {0.0, 0.0, 0.0};
+  // tiles come from the XCD's queue (dynamic schedule) or, for instances
+  // whose tiles are small, as the fixed sequence of elements vw / 8,
+  // vw / 8 + G / 8, ... of that queue (no atomic per tile)
+  __shared__ uint32_t s_pop;
+  const bool per_xcd = I.n_vwg % 8u == 0u;
+  const uint32_t qx = per_xcd ? vw % 8u : 0u;
+  const uint32_t e_stride = per_xcd ? I.n_vwg / 8u : I.n_vwg;
+  const gptr<uint32_t> queue = gmem(I.ctr) + kTileQueueCtr + qx;
+  uint32_t e = per_xcd ? vw / 8u : vw;
+  if (I.tile_dyn) {
+    if (tid == 0) s_pop = gatomic_add(queue, 1u);
+    __syncthreads();
+    e = s_pop;
+  }
+  const uint32_t j = tile_queue_pos(e, qx, per_xcd);
+  if (j >= I.ntiles) return;  // no tile: this workgroup's counters stay as they are
+  CollAcc cacc = {0ull, 0ull, 0ll};
   uint32_t tile = kInvalidKey;
-  if (j < I.ntiles) {
+  {
     tile = I.tile_order ? gmem(I.tile_order)[j] : j;
     const uint32_t c0 = tile * I.tile_cells;
     const uint32_t nc = min(I.tile_cells, I.ncells - c0);
@@ -977,36 +1004,40 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
     }
   }
   for (uint32_t b = 0; tile != kInvalidKey; b ^= 1u) {
-    w.e += w.step;
-    const uint32_t jn = tile_walk_pos(w);
-    const uint32_t tile_n = jn < I.ntiles ? (I.tile_order ? gmem(I.tile_order)[jn] : jn) : kInvalidKey;
+    uint32_t tile_n = kInvalidKey;  // set by tile_body
+    e += e_stride;                  // the next element of the fixed sequence
     const uint32_t c0 = tile * I.tile_cells;
     const uint32_t nc = min(I.tile_cells, I.ncells - c0);
     const uint32_t p0 = s_start[b][0];
     const uint32_t np = s_start[b][nc] - p0;
     if (np <= kLdsCap)
-      tile_body<MODE, true>(I, par, flags, tile, c0, nc, p0, np, s_start[b], s_nearc[b], tile_n, s_start[b ^ 1u],
+      tile_body<MODE, true>(I, par, flags, tile, c0, nc, p0, np, s_start[b], s_nearc[b], tile_n, queue, qx,
+                            per_xcd, e, &s_pop, s_start[b ^ 1u],
                             s_nearc[b ^ 1u], pf, s_raw, s_perm, s_owner, s_vx, s_vy, s_vz, stepno, ntl, cacc);
     else
-      tile_body<MODE, false>(I, par, flags, tile, c0, nc, p0, np, s_start[b], s_nearc[b], tile_n, s_start[b ^ 1u],
+      tile_body<MODE, false>(I, par, flags, tile, c0, nc, p0, np, s_start[b], s_nearc[b], tile_n, queue, qx,
+                            per_xcd, e, &s_pop, s_start[b ^ 1u],
                              s_nearc[b ^ 1u], pf, s_raw, s_perm, s_owner, s_vx, s_vy, s_vz, stepno, ntl, cacc);
     __syncthreads();
     tile = tile_n;
   }
 
-  // per-workgroup accumulators (deterministic: fixed reduction order)
-  double v[kAccSlots] = {cacc.sel, cacc.coll, cacc.cr, 0.0, 0.0, 0.0, 0.0, 0.0};
+  // per-workgroup collision counters (integers: exact in any order)
+  if (MODE == kModePhysics) {
+    long long v[3] = {(long long)cacc.sel, (long long)cacc.coll, cacc.cr};
+    long long* red = reinterpret_cast<long long*>(&s_red[0][0]);
 #pragma unroll
-  for (int k = 0; k < kAccSlots; ++k) {
-    double x = v[k];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-    if (lane_id() == 0) s_red[tid / 64][k] = x;
-  }
-  __syncthreads();
-  if (tid < kAccSlots) {
-    double t = 0.0;
-    for (uint32_t w = 0; w < kTileThreads / 64; ++w) t += s_red[w][tid];
-    gmem(I.wg_acc)[vw * kAccSlots + tid] += t;
+    for (int k = 0; k < 3; ++k) {
+      long long x = v[k];
+      for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+      if (lane_id() == 0) red[(tid / 64) * 3 + k] = x;
+    }
+    __syncthreads();
+    if (tid < 3) {
+      long long t = 0;
+      for (uint32_t w = 0; w < kTileThreads / 64; ++w) t += red[w * 3 + tid];
+      gmem(I.wg_acc)[vw * kAccSlots + tid] += t;
+    }
   }
 }
 

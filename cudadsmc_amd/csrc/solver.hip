@@ -62,6 +62,12 @@ struct dsmc_solver {
   void advance_time(uint64_t n) {
     for (uint64_t k = 0; k < n; ++k) sim_time += h.dt;
   }
+  // Expected particles per cell; tiles expecting at least 512 particles
+  // are taken from the tile queues, smaller ones in a fixed sequence: their
+  // phases are too short to hide the queue atomic, and it costs more than
+  // the balance gains (no result depends on the schedule).
+  double ppc_exp = 0.0;
+  void set_tile_schedule() { h.tile_dyn = (double)h.tile_cells * ppc_exp >= 512.0 ? 1u : 0u; }
   bool sampling = false;
   uint64_t sample_steps0 = 0;
   double acc_snap[kAccSlots] = {0};
@@ -301,6 +307,8 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
     while (tc > 32u && tc * ppc_exp > 0.5 * kLdsCap) tc >>= 1;
     I.tile_cells = tc;
     I.ntiles = (C + tc - 1) / tc;
+    h->ppc_exp = ppc_exp;
+    h->set_tile_schedule();
   }
   // RNG streams = 256 x the tile kernel's workgroups. By default one tile
   // per workgroup up to 1024 workgroups (small grids fill the GPU), else
@@ -880,16 +888,18 @@ int dsmc_get_fields(dsmc_solver* h, float* n, float* u, float* v, float* w, floa
 
 static int sum_acc(dsmc_solver* h, double out[kAccSlots]) {
   InstDesc& I = h->h;
-  std::vector<double> acc((size_t)I.n_vwg * kAccSlots);
+  std::vector<long long> acc((size_t)I.n_vwg * kAccSlots);
   HIPC(hipMemcpyAsync(acc.data(), I.wg_acc, acc.size() * 8, hipMemcpyDeviceToHost, h->stream));
   double tacc[4] = {0.0, 0.0, 0.0, 0.0};
   if (I.wall_acc) HIPC(hipMemcpyAsync(tacc, I.wall_acc, sizeof(tacc), hipMemcpyDeviceToHost, h->stream));
   HIPC(hipStreamSynchronize(h->stream));
+  // the workgroups' integer counters summed exactly (128-bit), converted once
   for (int k = 0; k < kAccSlots; ++k) {
-    double t = 0.0;
+    __int128 t = 0;
     for (uint32_t w = 0; w < I.n_vwg; ++w) t += acc[(size_t)w * kAccSlots + k];
-    out[k] = t;
+    out[k] = (double)t;
   }
+  out[kAccCr] /= kCrScale;
   // wall forces and heat of the wall pass
   for (int k = 0; k < 4; ++k) out[kAccFx + k] += tacc[k];
   return DSMC_OK;
@@ -1223,6 +1233,7 @@ int dsmc_set_tile_cells(dsmc_solver* h, uint32_t tile_cells) {
     return h->fail(DSMC_ERR_INVALID, "tile_cells must be a power of two in [32, %u]", kTileCells);
   h->h.tile_cells = tile_cells;
   h->h.ntiles = (h->h.ncells + tile_cells - 1) / tile_cells;
+  h->set_tile_schedule();
   return h->upload_desc();
 }
 

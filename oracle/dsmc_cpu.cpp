@@ -576,7 +576,7 @@ struct oracle_sim {
         if (sc > rmax) rmax = sc;
         if (sc > dsmc_rng::px_unit(d.v[0]) * max0) {
           a[1] += 1.0;
-          a[2] += mag;
+          a[2] += (double)std::llrint((double)mag * 65536.0);  // c_r in units of 2^-16 m/s, summed exactly
           scatter_uv(ux, uy, uz, i, j, mag, dsmc_rng::px_unit(d.v[1]), dsmc_rng::px_unit(d.v[2]));
         }
       }
@@ -1322,7 +1322,7 @@ int oracle_get_acc(oracle_sim* o, double* out, uint32_t n) {
   for (uint32_t k = 0; k < n && k < 7; ++k) {
     double t = 0;
     for (uint32_t w = 0; w < o->G; ++w) t += o->acc[(size_t)w * 7 + k];
-    out[k] = t;
+    out[k] = (k == 2 && o->phys) ? t / 65536.0 : t;  // PHYSICS c_r sums are in 2^-16 m/s
   }
   return 0;
 }
