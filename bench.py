@@ -109,18 +109,25 @@ def other_workload(name: str, warmup: int, steps: int) -> dict:
     return out
 
 
-def measured_traffic(workload: str):
-    """HBM bytes per k_tile launch of this workload as measured by rocprofv3
-    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a plain copy of
-    known size) and stored next to the profile summaries; None if absent."""
+def measured_traffic(workload: str, phase: str):
+    """HBM bytes per k_tile launch of this workload in the early ("early":
+    steps shortly after the freestream start) or developed ("developed":
+    after 1000 steps) flow, as measured by rocprofv3 (FETCH_SIZE +
+    WRITE_SIZE, scaled by the same counters on a plain copy of known size)
+    and stored next to the profile summaries; None if absent."""
     p = ROOT / "profiles" / f"k_tile_{workload}_traffic.txt"
     try:
         for line in p.read_text().splitlines():
-            if line.startswith("bytes_per_launch:"):
+            if line.startswith(f"{phase}_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
     except (OSError, ValueError):
         pass
     return None
+
+
+def traffic_phase(first_step: int) -> str:
+    """Which stored traffic figure fits timed steps that start at `first_step`."""
+    return "early" if first_step < 500 else "developed"
 
 
 def _oracle_case(name: str):
@@ -329,9 +336,11 @@ def main() -> int:
                                "walls": float(tm[6]), "inject": float(tm[3]), "total": float(tm[4]),
                                "timed_steps": int(tm[5])},
         "roofline": {"bound": "hbm", "achieved": tile_gbs / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": tile_gbs / HBM_PEAK, "traffic": measured_traffic(args.workload),
+                     "frac": tile_gbs / HBM_PEAK,
+                     "traffic": measured_traffic(args.workload, traffic_phase(args.warmup + 1)),
                      "traffic_source": f"stored: profiles/k_tile_{args.workload}_traffic.txt (rocprofv3 --pmc "
-                                       "FETCH_SIZE + WRITE_SIZE per k_tile launch, developed flow, copy-calibrated)",
+                                       f"FETCH_SIZE + WRITE_SIZE per k_tile launch, "
+                                       f"{traffic_phase(args.warmup + 1)} flow, copy-calibrated)",
                      "kernel": "k_tile (56 B/particle + 20 B/cell algorithmic)"},
         "validate_bad_particles": int(bad),
     }
@@ -343,6 +352,7 @@ def main() -> int:
             sol.synchronize()
         dev = throughput_line(sol, s, args.developed_steps, s.dt)
         dev["after_steps"] = max(done, args.developed_warmup)
+        dev["k_tile_traffic"] = measured_traffic(args.workload, "developed")
         line["developed_flow"] = dev
         sol.set_sampling(True)
         smp = throughput_line(sol, s, max(10, args.developed_steps // 2), s.dt, sampling=True)
