@@ -99,6 +99,7 @@ struct InstDesc {
   const uint32_t* cg_end;
   const uint32_t* cg_tri;    // entries: triangle index
   const float4* cg_rec;      // PHYSICS entries: {v0 + index bits in w, e1 = v1 - v0, e2 = v2 - v0, normal}
+  const uint2* cg_se;        // PHYSICS per cg cell: {first, end} of its entries (equal: not a surface cell)
   const float4* tv0;         // per triangle vertices and unit face normal (w unused)
   const float4* tv1;
   const float4* tv2;
@@ -173,6 +174,7 @@ __device__ __forceinline__ gptr<T> gmem(T* p) {
 typedef float dsmc_f2 __attribute__((ext_vector_type(2)));
 typedef float dsmc_f4 __attribute__((ext_vector_type(4)));
 typedef uint32_t dsmc_u4 __attribute__((ext_vector_type(4)));
+typedef uint32_t dsmc_u2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ float4 gload(gptr<const float4> p) {
   const dsmc_f4 v = *(gptr<const dsmc_f4>)p;
@@ -185,6 +187,10 @@ __device__ __forceinline__ float2 gload(gptr<const float2> p) {
 __device__ __forceinline__ uint4 gload(gptr<const uint4> p) {
   const dsmc_u4 v = *(gptr<const dsmc_u4>)p;
   return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 gload(gptr<const uint2> p) {
+  const dsmc_u2 v = *(gptr<const dsmc_u2>)p;
+  return make_uint2(v.x, v.y);
 }
 __device__ __forceinline__ void gstore(gptr<float4> p, float4 a) {
   *(gptr<dsmc_f4>)p = dsmc_f4{a.x, a.y, a.z, a.w};

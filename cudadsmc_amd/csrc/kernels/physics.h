@@ -275,77 +275,151 @@ __device__ __forceinline__ DdaAxis dda_axis(float o, float d, float t0, float lo
   return r;
 }
 
-constexpr uint32_t kRecBatch = 4;  // triangle records loaded together by the wall pass
+// The wall pass runs the DDA of a wavefront's 64 particles in lockstep: at
+// each step every live lane has one collision-grid cell, and the triangle
+// entries of all those cells are tested together, one entry per lane (the
+// entries of cell c of lane o are numbered after those of lanes < o), so a
+// cell with hundreds of entries costs a few wavefront-wide rounds rather
+// than hundreds of dependent loads on one lane. The nearest hit of each
+// lane in the step is the minimum of (s, triangle index) over its cell's
+// entries, kept as a 64-bit key in LDS (s >= 0, whose bits order as
+// integers; -0 is keyed as +0, as s == best treats them).
+struct WaveHitLds {
+  unsigned long long* key;  // [64] of the calling wavefront
+};
 
-__device__ __forceinline__ bool phys_nearest_hit(const InstDesc& I, float ox, float oy, float oz, float dx,
-                                                 float dy, float dz, float dist, uint32_t skip,
-                                                 float& s_hit, uint32_t& tri_hit) {
+__device__ __forceinline__ uint64_t hit_key(float t, uint32_t tri) {
+  return ((uint64_t)__float_as_uint(t + 0.0f) << 32) | tri;
+}
+
+// Entry k of a cell's records tested against the segment o + s d, s in
+// [0, dist]: front-facing, not the triangle just left.
+__device__ __forceinline__ bool test_entry(const InstDesc& I, uint32_t k, float ox, float oy, float oz, float dx,
+                                           float dy, float dz, float dist, uint32_t skip, float& t, uint32_t& tri) {
+  const gptr<const float4> rec = gmem<const float4>(I.cg_rec) + 4u * k;
+  const float4 r0 = gload(rec), r1 = gload(rec + 1), r2 = gload(rec + 2), n = gload(rec + 3);
+  tri = __float_as_uint(r0.w);
+  if (tri == skip) return false;
+  if (!(dx * n.x + dy * n.y + dz * n.z < 0.0f)) return false;
+  float u, v;
+  if (!intersect_edges<true>(ox, oy, oz, dx, dy, dz, r0, r1.x, r1.y, r1.z, r2.x, r2.y, r2.z, t, u, v)) return false;
+  return t >= 0.0f && t <= dist;
+}
+
+// Nearest front-facing triangle hit along o + s*d, 0 <= s <= dist, over the
+// collision-grid cells the segment crosses (3-D DDA), for the lanes with
+// `act`; ties on s go to the lower triangle index. Called by all 64 lanes
+// of the wavefront together.
+__device__ __forceinline__ bool wave_nearest_hit(const InstDesc& I, bool act, float ox, float oy, float oz,
+                                                 float dx, float dy, float dz, float dist, uint32_t skip,
+                                                 float& s_hit, uint32_t& tri_hit, const WaveHitLds& L) {
+  const uint32_t lane = __lane_id();
   float t0 = 0.0f, t1 = dist;
-  if (!dda_clip(ox, dx, I.cg_min[0], I.cg_max[0], t0, t1)) return false;
-  if (!dda_clip(oy, dy, I.cg_min[1], I.cg_max[1], t0, t1)) return false;
-  if (!dda_clip(oz, dz, I.cg_min[2], I.cg_max[2], t0, t1)) return false;
-  if (t0 > t1) return false;
-  DdaAxis ax = dda_axis(ox, dx, t0, I.cg_min[0], I.cg_cell[0], I.cg_dim[0]);
-  DdaAxis ay = dda_axis(oy, dy, t0, I.cg_min[1], I.cg_cell[1], I.cg_dim[1]);
-  DdaAxis az = dda_axis(oz, dz, t0, I.cg_min[2], I.cg_cell[2], I.cg_dim[2]);
+  bool on = act;
+  if (on) {
+    on = dda_clip(ox, dx, I.cg_min[0], I.cg_max[0], t0, t1) && dda_clip(oy, dy, I.cg_min[1], I.cg_max[1], t0, t1) &&
+         dda_clip(oz, dz, I.cg_min[2], I.cg_max[2], t0, t1) && !(t0 > t1);
+  }
+  DdaAxis ax = {0, 0, 0.0f, 0.0f}, ay = ax, az = ax;
+  if (on) {
+    ax = dda_axis(ox, dx, t0, I.cg_min[0], I.cg_cell[0], I.cg_dim[0]);
+    ay = dda_axis(oy, dy, t0, I.cg_min[1], I.cg_cell[1], I.cg_dim[1]);
+    az = dda_axis(oz, dz, t0, I.cg_min[2], I.cg_cell[2], I.cg_dim[2]);
+  }
   bool found = false;
   float best = 0.0f;
   uint32_t best_tri = 0;
   const int max_iter = (int)(I.cg_dim[0] + I.cg_dim[1] + I.cg_dim[2]) + 3;
-  for (int it = 0; it < max_iter; ++it) {
-    uint32_t cell = (uint32_t)az.c * I.cg_dim[0] * I.cg_dim[1] + (uint32_t)ay.c * I.cg_dim[0] + (uint32_t)ax.c;
-    if (gmem(I.cg_flag)[cell] == kCgSurface) {
-      uint32_t s = gmem(I.cg_start)[cell], e = gmem(I.cg_end)[cell];
-      // per-entry records {v0 + triangle id, e1, e2, n}: one contiguous
-      // 64-byte read per candidate, no dependent index loads
-      gptr<const float4> rec = gmem<const float4>(I.cg_rec) + 4u * s;
-      // kRecBatch records in flight before any is tested
-      for (uint32_t i = s; i < e; i += kRecBatch, rec += 4 * kRecBatch) {
-        float4 ra[kRecBatch][4];
+  int it = 0;
+  while (__ballot(on) != 0ull) {
+    // this step's cell of each live lane and its entry range
+    uint32_t s = 0u, n = 0u;
+    if (on) {
+      const uint32_t cell =
+          (uint32_t)az.c * I.cg_dim[0] * I.cg_dim[1] + (uint32_t)ay.c * I.cg_dim[0] + (uint32_t)ax.c;
+      const uint2 se = gload(gmem<const uint2>(I.cg_se) + cell);
+      s = se.x;
+      n = se.y - se.x;
+    }
+    uint32_t incl = n;
 #pragma unroll
-        for (uint32_t k = 0; k < kRecBatch; ++k) {
-          if (i + k < e) {
+    for (uint32_t o = 1; o < 64u; o <<= 1) {
+      const uint32_t x = __shfl_up(incl, o);
+      if (lane >= o) incl += x;
+    }
+    const uint32_t off = incl - n;
+    const uint32_t total = __shfl(incl, 63);
+    if (total != 0u) {  // uniform over the wavefront
+      if (n) L.key[lane] = ~0ull;
+      // the keys are set before any lane's minimum reaches them
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      for (uint32_t k0 = 0; k0 < total; k0 += 64u) {
+        const uint32_t k = k0 + lane < total ? k0 + lane : total - 1u;
+        // owner: the last lane whose first entry number is <= k
+        uint32_t o = 0;
 #pragma unroll
-            for (int w = 0; w < 4; ++w) ra[k][w] = gload(rec + 4 * k + w);
-          }
+        for (uint32_t step = 32; step; step >>= 1) {
+          const uint32_t m = o + step;
+          const uint32_t om = __shfl(off, (int)(m & 63u));
+          if (m < 64u && om <= k) o = m;
         }
-#pragma unroll
-        for (uint32_t k = 0; k < kRecBatch; ++k) {
-          if (i + k >= e) break;
-          const float4 r0 = ra[k][0], r1 = ra[k][1], r2 = ra[k][2], n = ra[k][3];
-          const uint32_t tri = __float_as_uint(r0.w);
-          if (tri == skip) continue;
-          if (!(dx * n.x + dy * n.y + dz * n.z < 0.0f)) continue;
-          float t, u, v;
-          if (intersect_edges<true>(ox, oy, oz, dx, dy, dz, r0, r1.x, r1.y, r1.z, r2.x, r2.y, r2.z, t, u, v)) {
-            if (t >= 0.0f && t <= dist) {
-              if (!found || t < best || (t == best && tri < best_tri)) {
-                found = true;
-                best = t;
-                best_tri = tri;
+        const uint32_t e = __shfl(s, (int)o) + (k - __shfl(off, (int)o));
+        const float qx = __shfl(ox, (int)o), qy = __shfl(oy, (int)o), qz = __shfl(oz, (int)o);
+        const float ex = __shfl(dx, (int)o), ey = __shfl(dy, (int)o), ez = __shfl(dz, (int)o);
+        const float qd = __shfl(dist, (int)o);
+        const uint32_t qs = __shfl(skip, (int)o);
+        float t;
+        uint32_t tri;
+        if (k0 + lane < total && test_entry(I, e, qx, qy, qz, ex, ey, ez, qd, qs, t, tri))
+          atomicMin(&L.key[o], hit_key(t, tri));
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      if (n) {
+        const unsigned long long key = L.key[lane];
+        if (key != ~0ull) {
+          const float t = __uint_as_float((uint32_t)(key >> 32));
+          const uint32_t tri = (uint32_t)key;
+          if (!found || t < best || (t == best && tri < best_tri)) {
+            found = true;
+            best = t;
+            best_tri = tri;
+            if (t == 0.0f) {
+              // the sign of a zero s as computed (the key holds +0)
+              for (uint32_t q = s; q < s + n; ++q) {
+                float tq;
+                uint32_t trq;
+                if (test_entry(I, q, ox, oy, oz, dx, dy, dz, dist, skip, tq, trq) && trq == tri) {
+                  best = tq;
+                  break;
+                }
               }
             }
           }
         }
       }
     }
-    bool xmin = ax.tmax < ay.tmax ? ax.tmax < az.tmax : false;
-    bool ymin = !xmin && (ay.tmax < az.tmax);
-    float texit = xmin ? ax.tmax : (ymin ? ay.tmax : az.tmax);
-    if (found && best <= texit) break;
-    if (texit > t1) break;
-    if (xmin) {
-      ax.c += ax.step;
-      if (ax.c < 0 || ax.c >= (int)I.cg_dim[0]) break;
-      ax.tmax = ax.tmax + ax.tdel;
-    } else if (ymin) {
-      ay.c += ay.step;
-      if (ay.c < 0 || ay.c >= (int)I.cg_dim[1]) break;
-      ay.tmax = ay.tmax + ay.tdel;
-    } else {
-      az.c += az.step;
-      if (az.c < 0 || az.c >= (int)I.cg_dim[2]) break;
-      az.tmax = az.tmax + az.tdel;
+    if (on) {
+      const bool xmin = ax.tmax < ay.tmax ? ax.tmax < az.tmax : false;
+      const bool ymin = !xmin && (ay.tmax < az.tmax);
+      const float texit = xmin ? ax.tmax : (ymin ? ay.tmax : az.tmax);
+      if ((found && best <= texit) || texit > t1) {
+        on = false;
+      } else if (xmin) {
+        ax.c += ax.step;
+        ax.tmax = ax.tmax + ax.tdel;
+        on = ax.c >= 0 && ax.c < (int)I.cg_dim[0];
+      } else if (ymin) {
+        ay.c += ay.step;
+        ay.tmax = ay.tmax + ay.tdel;
+        on = ay.c >= 0 && ay.c < (int)I.cg_dim[1];
+      } else {
+        az.c += az.step;
+        az.tmax = az.tmax + az.tdel;
+        on = az.c >= 0 && az.c < (int)I.cg_dim[2];
+      }
+      if (++it >= max_iter) on = false;
     }
   }
   s_hit = best;
@@ -434,21 +508,37 @@ __device__ __forceinline__ uint32_t move_free(const InstDesc& I, float& px, floa
 // specular), up to kPhysMaxBounces, then the box faces. The diffuse
 // re-emission of bounce b draws Philox4x32-10 at counter (slot, step, 0, b)
 // under key (seed, instance salt): the draws depend on which particle
-// bounces when, not on which thread handles it.
-__device__ __forceinline__ uint32_t move_walls(const InstDesc& I, float& px, float& py, float& pz, float& vx,
-                                               float& vy, float& vz, float dt, uint32_t slot, uint32_t stepno,
-                                               uint32_t& hits, WallAcc& acc) {
+// bounces when, not on which thread handles it. Called by all 64 lanes of
+// the wavefront together; lanes without a particle pass act = false (and
+// get kInvalidKey).
+__device__ __forceinline__ uint32_t move_walls(const InstDesc& I, bool act, float& px, float& py, float& pz,
+                                               float& vx, float& vy, float& vz, float dt, uint32_t slot,
+                                               uint32_t stepno, uint32_t& hits, WallAcc& acc, const WaveHitLds& L) {
   hits = 0;
   float tl = dt;
   uint32_t skip = kInvalidKey;
+  bool fly = act;
   for (int b = 0; b < kPhysMaxBounces; ++b) {
-    float sp = sqrtf(vx * vx + vy * vy + vz * vz);
-    float dist = sp * tl;
-    if (!(dist > 0.0f)) break;
-    float dx = vx / sp, dy = vy / sp, dz = vz / sp;
+    float sp = 0.0f, dist = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    if (fly) {
+      sp = sqrtf(vx * vx + vy * vy + vz * vz);
+      dist = sp * tl;
+      fly = dist > 0.0f;
+      if (fly) {
+        dx = vx / sp;
+        dy = vy / sp;
+        dz = vz / sp;
+      }
+    }
+    if (__ballot(fly) == 0ull) break;  // uniform over the wavefront
     float s;
     uint32_t tri;
-    if (!phys_nearest_hit(I, px, py, pz, dx, dy, dz, dist, skip, s, tri)) break;
+    const bool hit = wave_nearest_hit(I, fly, px, py, pz, dx, dy, dz, dist, skip, s, tri, L);
+    if (!fly) continue;
+    if (!hit) {
+      fly = false;
+      continue;
+    }
     float4 n = gload(gmem<const float4>(I.tn) + tri);
     float hx = px + dx * s, hy = py + dy * s, hz = pz + dz * s;
     float ux, uy, uz;
@@ -488,6 +578,7 @@ __device__ __forceinline__ uint32_t move_walls(const InstDesc& I, float& px, flo
     vz = uz;
     skip = tri;
   }
+  if (!act) return kInvalidKey;
   if (tl > 0.0f) {
     px = px + vx * tl;
     py = py + vy * tl;

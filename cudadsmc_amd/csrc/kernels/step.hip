@@ -1109,7 +1109,8 @@ __global__ void __launch_bounds__(kTileThreads) k_defer_list(const InstDesc* __r
 }
 
 // k_walls moves the listed particles, one per thread (a grid-stride loop
-// over the list). The pass holds no RNG state (Philox draws keyed by slot
+// over the list); the 64 lanes of a wavefront fly their particles together,
+// testing the triangle entries of all their cells as one pool. The pass holds no RNG state (Philox draws keyed by slot
 // and step). Wall forces and heat are summed in 64-bit fixed point, so the
 // order in which particles are listed and run does not change the sums;
 // each workgroup stores its sums in its own slot of wall_part, and
@@ -1119,6 +1120,7 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_walls(const InstDesc* __res
   const InstDesc& I = Ip[blockIdx.y];
   if (!I.defer_bits || blockIdx.x >= I.n_wall_wg) return;
   __shared__ long long s_red[kTileThreads / 64][4];
+  __shared__ unsigned long long s_hit[kTileThreads / 64][64];
   const uint32_t cnt = I.ctr[5];
   const uint32_t stepno = I.ctr[4];
   const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / 64;
@@ -1131,17 +1133,23 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_walls(const InstDesc* __res
   const gptr<uint32_t> key = gmem(I.key[opar]);
   const gptr<const uint32_t> list = gmem<const uint32_t>(I.defer_list);
   WallAcc acc = {0, 0, 0, 0};
-  // uniform trip count over the workgroup: the counting is wave-aggregated
+  const WaveHitLds hl = {s_hit[wid]};
+  // uniform trip count over the workgroup: the wall flights run the
+  // wavefront's lanes together, and the counting is wave-aggregated
   // (neighbouring list entries mostly land in the same cell)
   for (uint32_t b = blockIdx.x * kTileThreads; b < cnt; b += I.n_wall_wg * kTileThreads) {
     const uint32_t i = b + tid;
-    uint32_t k = kInvalidKey, slot = 0;
-    if (i < cnt) {
+    const bool act = i < cnt;
+    uint32_t slot = 0;
+    float x = 0.0f, y = 0.0f, z = 0.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    if (act) {
       slot = list[i];
-      float x = px[slot], y = py[slot], z = pz[slot];
-      float ux = vx[slot], uy = vy[slot], uz = vz[slot];
-      uint32_t hits;
-      k = move_walls(I, x, y, z, ux, uy, uz, I.dt, slot, stepno, hits, acc);
+      x = px[slot], y = py[slot], z = pz[slot];
+      ux = vx[slot], uy = vy[slot], uz = vz[slot];
+    }
+    uint32_t hits;
+    const uint32_t k = move_walls(I, act, x, y, z, ux, uy, uz, I.dt, slot, stepno, hits, acc, hl);
+    if (act) {
       px[slot] = x;
       py[slot] = y;
       pz[slot] = z;

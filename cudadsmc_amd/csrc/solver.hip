@@ -533,6 +533,7 @@ int dsmc_set_geometry_moved(dsmc_solver* h, const float* verts, uint32_t nv, con
   HIPC(hipMemcpyAsync(t2, h2.data(), nt * sizeof(float4), hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(tn, hn.data(), nt * sizeof(float4), hipMemcpyHostToDevice, st));
   I.cg_rec = nullptr;
+  I.cg_se = nullptr;
   if (h->physics()) {
     const size_t ne = std::max<size_t>(g.tri.size(), 1);
     float4* rec;
@@ -550,8 +551,16 @@ int dsmc_set_geometry_moved(dsmc_solver* h, const float* verts, uint32_t nv, con
       hr[4 * i + 3] = hn[t];
     }
     HIPC(hipMemcpyAsync(rec, hr.data(), hr.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    // entry ranges of the surface cells, one 8-byte load per cell visited
+    uint2* se;
+    CHECK_RC(h->alloc(&se, g.ncells(), true));
+    std::vector<uint2> hse(g.ncells(), make_uint2(0u, 0u));
+    for (size_t c = 0; c < g.ncells(); ++c)
+      if (g.flag[c] == kCgSurface) hse[c] = make_uint2(g.start[c], g.end[c]);
+    HIPC(hipMemcpyAsync(se, hse.data(), hse.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
     HIPC(hipStreamSynchronize(st));
     I.cg_rec = rec;
+    I.cg_se = se;
   }
   for (int a = 0; a < 3; ++a) {
     I.cg_min[a] = g.mn[a];
