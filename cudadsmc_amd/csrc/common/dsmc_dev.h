@@ -5,7 +5,9 @@
 //   keys                      : uint32[cap] per buffer (cell of the particle,
 //                               0xFFFFFFFF = removed)
 //   count                     : uint32[ncells]      next-step counting
-//   rank                      : uint32[cap]         place of each stored particle in its cell's count
+//   cursor                    : uint32[ncells]      scatter: next free sorted position of each cell
+//   local_cnt                 : uint32[ncells]      particles moved into the cell from its own tile
+//   tile_p0                   : uint32[ntiles + 1]  first stored slot of each tile's particles
 //   cell_start                : uint32[ncells + 1]  offsets of the current order
 //   perm / perm_tmp           : uint32[cap]         sorted position -> storage index
 //   ntc                       : float2[ncells]      {(sigma c_r)max, remainder}
@@ -21,6 +23,10 @@ namespace dsmc_dev {
 
 constexpr uint32_t kInvalidKey = 0xFFFFFFFFu;
 constexpr uint32_t kDeferredKey = 0xFFFFFFFEu;  // PHYSICS: moved by the wall pass
+// A key the wall pass stored carries this bit, so that the scatter does not
+// take the particle for one that stayed in its tile (cells < 2^31).
+constexpr uint32_t kWallKeyBit = 0x80000000u;
+__host__ __device__ inline uint32_t key_cell(uint32_t k) { return k >= kDeferredKey ? kInvalidKey : (k & ~kWallKeyBit); }
 constexpr uint32_t kTileCells = 256;    // most cells per tile (one lane per cell)
 constexpr uint32_t kTileThreads = 256;  // threads of the tile kernel
 constexpr uint32_t kLdsCap = 2048;      // particles staged in LDS per tile
@@ -69,7 +75,8 @@ struct InstDesc {
   uint32_t ncells;
   uint32_t tile_cells;    // cells per tile: a power of two in [32, kTileCells]
   uint32_t ntiles;
-  uint32_t tile_dyn;      // tile kernel takes tiles from atomic queues (else a fixed sequence)
+  uint32_t tile_dyn;      // large tiles: the tile kernel takes tiles from atomic queues (else a fixed
+                          // sequence) and counts the particles that stay in their tile for the scatter
   float cell_volume;
   // ---- gas and NTC
   float mass;
@@ -117,7 +124,11 @@ struct InstDesc {
   uint32_t* count;
   uint32_t* cell_start;
   uint32_t* perm;
-  uint32_t* rank;       // per stored slot: its place among the particles counted into its cell [cap]
+  uint32_t* cursor;     // per cell: next free position of the scatter for particles from other tiles
+                        // (the scan sets it to cell_start + local_cnt)
+  uint32_t* local_cnt;  // per cell: particles the tile kernel moved into it from the cell's own tile
+  uint32_t* tile_p0;    // [ntiles + 1]: slot of the first particle the tile kernel stored for each tile;
+                        // [ntiles] = end of the tile-ordered slots (all zero: no tile layout yet)
   uint32_t* perm_tmp;   // overflow tiles: sorted copy of the segment
   uint8_t* owner_tmp;   // overflow tiles: cell of each position within the tile
   float2* ntc;

@@ -52,13 +52,13 @@ __device__ __forceinline__ void mt_store(const InstDesc& I, uint32_t s, const Mt
   for (int w = 0; w < kMtN; ++w) gmem(I.mt_state)[(uint32_t)w * I.n_streams + s] = m.st[w];
 }
 
-// Wave-aggregated atomicAdd(&cnt[key], 1) for keys of consecutive lanes and
-// the lane's rank among the particles counted into its cell (the count
-// before it): one atomic per run of equal keys. Invalid keys are skipped
-// (rank 0).
-// Split in two so that callers can issue the atomics of several rounds
-// before waiting for any returned count: count_issue returns the head's
-// count (0 elsewhere) and the wave's run heads, count_finish the rank.
+// Wave-aggregated atomicAdd(&cnt[key], 1) for keys of consecutive lanes,
+// returning each lane's count before its own increment: one atomic per run
+// of equal keys, whose lanes take consecutive values. Invalid keys are
+// skipped (their result is meaningless). Split in two so that a caller can
+// issue the atomics of several rounds before waiting for any: count_issue
+// returns the run head's old count (0 elsewhere) and the wave's run heads,
+// count_finish the lane's value.
 __device__ __forceinline__ uint32_t count_issue(gptr<uint32_t> cnt, uint32_t key, uint64_t& heads) {
   const uint32_t lane = lane_id();
   const uint32_t prev = __shfl_up(key, 1);
@@ -79,12 +79,6 @@ __device__ __forceinline__ uint32_t count_finish(uint32_t base, uint64_t heads) 
   const uint64_t below_incl = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
   const uint32_t hl = 63u - (uint32_t)__clzll((long long)(heads & below_incl));
   return __shfl(base, (int)hl) + (lane - hl);
-}
-
-__device__ __forceinline__ uint32_t count_rank(gptr<uint32_t> cnt, uint32_t key) {
-  uint64_t heads;
-  const uint32_t base = count_issue(cnt, key, heads);
-  return count_finish(base, heads);
 }
 
 // Wave-aggregated atomicAdd(&cnt[key], 1) for keys of consecutive lanes:
@@ -149,8 +143,7 @@ __global__ void k_keys_count(const InstDesc* __restrict__ Ip, uint32_t par, uint
     key = I.mode == kModePhysics ? cell_key_phys(I, x, y, z) : cell_key(I, x, y, z);
     I.key[par][i] = key;
   }
-  const uint32_t rk = count_rank(gmem(I.count), key);
-  if (i < n) gmem(I.rank)[i] = rk;
+  count_keys(gmem(I.count), key);
 }
 
 // Plain counting of arbitrary keys (sort operator); keys >= n_cells are
@@ -365,7 +358,10 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __
   // all lie below n moves them as 16-byte vectors when the arrays are 16-byte
   // aligned (the engine's are 256-byte aligned; base is a multiple of 16)
   uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  const bool whole = base + kScanItems <= n && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0u;
+  const bool whole = base + kScanItems <= n &&
+                     ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out) |
+                       (which ? 0u : (reinterpret_cast<uintptr_t>(I.cursor) | reinterpret_cast<uintptr_t>(I.local_cnt)))) &
+                      15u) == 0u;
   uint32_t v[kScanItems];
   if (whole) {
     const gptr<const uint4> src = gmem<const uint4>(reinterpret_cast<const uint4*>(in + base));
@@ -381,6 +377,13 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k) v[k] = base + k < n ? in[base + k] : 0u;
   }
+  // the cells' counts of particles that stayed in their tile (cursor =
+  // offset + count), loaded before the scan's barriers
+  uint4 l[kScanItems / 4u];
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems / 4u; ++k)
+    l[k] = (which == 0 && whole) ? gload(gmem<const uint4>(reinterpret_cast<const uint4*>(I.local_cnt + base)) + k)
+                                 : make_uint4(0u, 0u, 0u, 0u);
   uint32_t local = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kScanItems; ++k) local += v[k];
@@ -394,7 +397,11 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __
       const uint32_t o0 = off, o1 = o0 + v[4 * k], o2 = o1 + v[4 * k + 1], o3 = o2 + v[4 * k + 2];
       off = o3 + v[4 * k + 3];
       gstore(dst + k, make_uint4(o0, o1, o2, o3));
-      if (which == 0) gstore(clr + k, make_uint4(0u, 0u, 0u, 0u));
+      if (which == 0) {
+        gstore(clr + k, make_uint4(0u, 0u, 0u, 0u));
+        gstore(gmem(reinterpret_cast<uint4*>(I.cursor + base)) + k,
+               make_uint4(o0 + l[k].x, o1 + l[k].y, o2 + l[k].z, o3 + l[k].w));
+      }
     }
   } else {
 #pragma unroll
@@ -402,7 +409,10 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __
       uint32_t i = base + k;
       if (i < n) {
         out[i] = off;
-        if (which == 0) in[i] = 0u;
+        if (which == 0) {
+          in[i] = 0u;
+          I.cursor[i] = off + I.local_cnt[i];
+        }
       }
       off += v[k];
     }
@@ -411,23 +421,83 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __
 }
 
 // =================================================================== scatter
-// perm[cell_start[key[i]] + rank[i]] = i: the rank of each stored particle
-// among those counted into its cell was kept by the kernel that counted it,
-// so no atomics are needed here. The order within a cell is the counting
-// order; the tile kernel sorts each cell's segment by storage index.
+// Places every stored particle i in the sorted order: perm[dst] = i, dst in
+// its cell's segment. The tile kernel stored the particles of each tile in
+// one run of slots (tile_p0) and counted per cell how many of them stayed
+// in the tile's own cells (local_cnt); the scan set cursor = cell_start +
+// local_cnt. A workgroup takes kScatterSpan consecutive cells (whole
+// tiles) and the slots of their tiles: the particles that stayed in their
+// tile are ranked with LDS atomics into the first local_cnt places of their
+// cells; the others (they left their tile, went through the wall pass, or
+// were injected after the tile-ordered slots) take the places from the
+// cursor on, with one wave-aggregated global atomic per run of equal keys
+// and the atomics of kScatterRounds rounds of 256 slots in flight together.
+// The order within a cell is the order in which the atomics landed; the
+// tile kernel sorts each cell's segment by storage index.
+constexpr uint32_t kScatterRounds = 4;
+constexpr uint32_t kScatterSpan = 1024;  // cells per workgroup pass (a multiple of every tile size)
+
 __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip, uint32_t par) {
   const InstDesc& I = Ip[blockIdx.y];
+  __shared__ uint32_t s_cnt[kScatterSpan];
+  __shared__ uint32_t s_tp[kScatterSpan / 32u + 1u];
+  const uint32_t tid = threadIdx.x;
   const uint32_t n = I.ctr[0];
   const gptr<const uint32_t> keys = gmem<const uint32_t>(I.key[par]);
-  const gptr<const uint32_t> rank = gmem<const uint32_t>(I.rank);
   const gptr<const uint32_t> start = gmem<const uint32_t>(I.cell_start);
+  const gptr<const uint32_t> tp0 = gmem<const uint32_t>(I.tile_p0);
   const gptr<uint32_t> perm = gmem(I.perm);
-  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
-    const uint32_t key = keys[i];
-    if (key == kInvalidKey) continue;
-    const uint32_t dst = start[key] + rank[i];
-    if (dst < I.cap) perm[dst] = i;
+  const gptr<uint32_t> cursor = gmem(I.cursor);
+  // end of the tile-ordered slots; instances with small tiles place every
+  // particle through the cursors (their tile kernel counts none as stayed)
+  const uint32_t S = I.tile_dyn ? min(tp0[I.ntiles], n) : 0u;
+  const uint32_t K = kScatterSpan / I.tile_cells;  // tiles per pass
+  // rounds of 256 slots from q0 (below q1); tiles [t0, t0 + K) when K > 0,
+  // whose slot runs start at s_tp[] and whose cells' next places are s_cnt[]
+  auto place = [&](uint32_t q0, uint32_t q1, uint32_t t0, uint32_t K) {
+    const uint32_t c0 = t0 * I.tile_cells;
+    uint32_t key[kScatterRounds], base[kScatterRounds], loc[kScatterRounds];
+    uint64_t heads[kScatterRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kScatterRounds; ++r) {
+      const uint32_t i = q0 + r * 256u + tid;
+      key[r] = i < q1 ? keys[i] : kInvalidKey;
+      // stayed: its cell lies in the tile whose slots hold it
+      loc[r] = kInvalidKey;
+      const uint32_t rel = key[r] - c0;
+      if (K && rel < K * I.tile_cells) {
+        uint32_t t = 0;
+        for (uint32_t h = K; h > 1u;) {  // the last tile whose first slot is <= i
+          const uint32_t half = h >> 1;
+          if (s_tp[t + half] <= i) t += half;
+          h -= half;
+        }
+        if (rel / I.tile_cells == t) loc[r] = rel;
+      }
+      const uint32_t g = loc[r] != kInvalidKey ? kInvalidKey : key_cell(key[r]);
+      base[r] = count_issue(cursor, g, heads[r]);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kScatterRounds; ++r) {
+      const uint32_t i = q0 + r * 256u + tid;
+      uint32_t dst = count_finish(base[r], heads[r]);
+      if (loc[r] != kInvalidKey) dst = atomicAdd(&s_cnt[loc[r]], 1u);
+      if (key_cell(key[r]) != kInvalidKey && dst < I.cap) perm[dst] = i;
+    }
+  };
+  const uint32_t passes = S ? (I.ntiles + K - 1u) / K : 0u;
+  for (uint32_t g = blockIdx.x; g < passes; g += gridDim.x) {  // uniform over the workgroup
+    const uint32_t t0 = g * K;
+    // the next place of each cell for particles that stayed: from its start
+    for (uint32_t c = tid; c < kScatterSpan; c += 256u) s_cnt[c] = t0 * I.tile_cells + c < I.ncells ? start[t0 * I.tile_cells + c] : 0u;
+    if (tid <= K) s_tp[tid] = t0 + tid <= I.ntiles ? min(tp0[t0 + tid], S) : S;
+    __syncthreads();
+    const uint32_t p0 = s_tp[0], p1 = max(s_tp[K], p0);
+    for (uint32_t q0 = p0; q0 < p1; q0 += 256u * kScatterRounds) place(q0, p1, t0, K);
+    __syncthreads();  // s_cnt and s_tp are set again for the next pass
   }
+  for (uint32_t q0 = S + blockIdx.x * 256u * kScatterRounds; q0 < n; q0 += gridDim.x * 256u * kScatterRounds)
+    place(q0, n, 0u, 0u);
 }
 
 // ============================================================ sort operator
@@ -804,6 +874,10 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
     if (tid == 0) sn_start[ncn] = nx_end;
     if (MODE == kModePhysics) sn_nearc[tid] = nx_near;
   }
+  // per-cell counts of the particles that stay in the tile (the NTC is done
+  // with the offsets that share this LDS)
+  ntl.off[tid] = 0u;
+  if (tid == 0) gmem(I.tile_p0)[tile] = p0;
   __syncthreads();
   // the next tile's first permutation entries (its offsets are in LDS now)
   if (MODE == kModePhysics && tile_n != kInvalidKey) {
@@ -822,9 +896,6 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
     // positions of up to kMoveBatch rounds in flight before the first store
     float bx[kMoveBatch], by[kMoveBatch], bz[kMoveBatch];
     uint32_t bsrc[kMoveBatch];
-    // counting atomics of the batch: ranks read after the last round
-    uint32_t cbase[kMoveBatch];
-    uint64_t cheads[kMoveBatch];
 #pragma unroll
     for (uint32_t k = 0; k < kMoveBatch; ++k) {
       const uint32_t q = (r0 + k) * kTileThreads + tid;
@@ -873,7 +944,8 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
       if (flags & kFlagHits) gmem(I.hits)[p0 + q] = hits;
       if (flags & kFlagKeepPerm) gmem(I.perm)[p0 + q] = src;
     }
-    cbase[k] = count_issue(gmem(I.count), key == kDeferredKey ? kInvalidKey : key, cheads[k]);
+    count_keys(gmem(I.count), key == kDeferredKey ? kInvalidKey : key);
+    if (I.tile_dyn && key - c0 < nc) atomicAdd(&ntl.off[key - c0], 1u);  // stayed in the tile's cells
     if (MODE == kModePhysics) {
       // mark the wave's deferred slots in the bitmask the wall pass reads
       const uint64_t m = __ballot(key == kDeferredKey);
@@ -887,13 +959,6 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
                                 __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < kMoveBatch; ++k) {
-      if (r0 + k >= rounds) break;  // uniform over the workgroup
-      const uint32_t q = (r0 + k) * kTileThreads + tid;
-      const uint32_t rk = count_finish(cbase[k], cheads[k]);
-      if (q < np) gmem(I.rank)[p0 + q] = rk;
     }
   }
 
@@ -964,7 +1029,10 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
   const uint32_t tid = threadIdx.x;
   const uint32_t vw = blockIdx.x;
   const uint32_t stepno = gmem(I.ctr)[4];  // this step's number (counter of the PHYSICS draws)
-  if (vw == 0 && tid == 0) I.ctr[0] = I.ctr[1];
+  if (vw == 0 && tid == 0) {
+    I.ctr[0] = I.ctr[1];
+    I.tile_p0[I.ntiles] = I.ctr[1];  // the slots this launch stores in tile order
+  }
   // tiles come from the XCD's queue (dynamic schedule) or, for instances
   // whose tiles are small, as the fixed sequence of elements vw / 8,
   // vw / 8 + G / 8, ... of that queue (no atomic per tile)
@@ -1019,6 +1087,7 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
                             per_xcd, e, &s_pop, s_start[b ^ 1u],
                              s_nearc[b ^ 1u], pf, s_raw, s_perm, s_owner, s_vx, s_vy, s_vz, stepno, ntl, cacc);
     __syncthreads();
+    if (tid < nc) gmem(I.local_cnt)[c0 + tid] = s_ntc_off[tid];
     tile = tile_n;
   }
 
@@ -1156,11 +1225,10 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_walls(const InstDesc* __res
       vx[slot] = ux;
       vy[slot] = uy;
       vz[slot] = uz;
-      key[slot] = k;
+      key[slot] = k == kInvalidKey ? k : (k | kWallKeyBit);
       if (flags & kFlagHits) gmem(I.hits)[slot] = hits;
     }
-    const uint32_t rk = count_rank(gmem(I.count), k);
-    if (k != kInvalidKey) gmem(I.rank)[slot] = rk;
+    count_keys(gmem(I.count), k);
   }
   long long v[4] = {acc.fx, acc.fy, acc.fz, acc.heat};
 #pragma unroll
@@ -1477,7 +1545,7 @@ __global__ void k_inject_generate(const InstDesc* __restrict__ Ip, uint32_t opar
     I.vx[opar][idx] = v[0]; I.vy[opar][idx] = v[1]; I.vz[opar][idx] = v[2];
     const uint32_t key = cell_key_phys(I, x, y, zz);
     I.key[opar][idx] = key;
-    gmem(I.rank)[idx] = gatomic_add(gmem(I.count) + key, 1u);
+    gatomic_add(gmem(I.count) + key, 1u);
   }
 }
 

@@ -65,7 +65,10 @@ struct dsmc_solver {
   // Expected particles per cell; tiles expecting at least 512 particles
   // are taken from the tile queues, smaller ones in a fixed sequence: their
   // phases are too short to hide the queue atomic, and it costs more than
-  // the balance gains (no result depends on the schedule).
+  // the balance gains. The same flag has the scatter rank the particles
+  // that stayed in their tile in LDS (with small tiles, the per-pass work
+  // costs more than the global atomics it saves). No result depends on
+  // either choice.
   double ppc_exp = 0.0;
   void set_tile_schedule() { h.tile_dyn = (double)h.tile_cells * ppc_exp >= 512.0 ? 1u : 0u; }
   bool sampling = false;
@@ -351,7 +354,8 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
     CHECK_RC(h->alloc(&I.key[b], cap));
   }
   CHECK_RC(h->alloc(&I.perm, cap));
-  CHECK_RC(h->alloc(&I.rank, cap));
+  CHECK_RC(h->alloc(&I.cursor, C));
+  CHECK_RC(h->alloc(&I.local_cnt, C));
   CHECK_RC(h->alloc(&I.perm_tmp, cap));
   CHECK_RC(h->alloc(&I.owner_tmp, cap));
   CHECK_RC(h->alloc(&I.hits, cap));
@@ -638,6 +642,7 @@ static int finish_loading(dsmc_solver* h, uint32_t n, uint32_t* n_out) {
     HIPC(hipMemcpyAsync(d, order.data(), (size_t)I.ntiles * 4, hipMemcpyHostToDevice, st));
     HIPC(hipStreamSynchronize(st));
     I.tile_order = d;
+    CHECK_RC(h->alloc(&I.tile_p0, (size_t)I.ntiles + 1));
     CHECK_RC(h->upload_desc());
   }
   if (h->physics() && I.has_geom && !I.defer_bits) {
@@ -667,6 +672,9 @@ static int finish_loading(dsmc_solver* h, uint32_t n, uint32_t* n_out) {
   uint32_t ctr[2] = {n, n};
   HIPC(hipMemcpyAsync(I.ctr, ctr, 8, hipMemcpyHostToDevice, st));
   HIPC(hipMemsetAsync(I.count, 0, (size_t)I.ncells * 4, st));
+  // no tile layout yet: the first scatter places every particle through the cursors
+  HIPC(hipMemsetAsync(I.local_cnt, 0, (size_t)I.ncells * 4, st));
+  HIPC(hipMemsetAsync(I.tile_p0, 0, ((size_t)I.ntiles + 1) * 4, st));
   dsmc_launch::keys_count(h->d_desc, h->par, n, st);
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(st));
@@ -1146,6 +1154,8 @@ int dsmc_get_particles(dsmc_solver* h, float* pos, float* vel, uint32_t* keys, u
   }
   if (keys) HIPC(hipMemcpyAsync(keys, I.key[p], (size_t)n * 4, hipMemcpyDeviceToHost, st));
   HIPC(hipStreamSynchronize(st));
+  if (keys)
+    for (uint32_t i = 0; i < n; ++i) keys[i] = key_cell(keys[i]);  // without the wall pass's mark
   return DSMC_OK;
 }
 
@@ -1237,7 +1247,8 @@ int dsmc_set_rng_state(dsmc_solver* h, const uint32_t* in, uint32_t n) {
 int dsmc_set_tile_cells(dsmc_solver* h, uint32_t tile_cells) {
   if (!h) return set_global_error(DSMC_ERR_INVALID, "null handle");
   if (!h->inited) return h->fail(DSMC_ERR_STATE, "not initialised");
-  if (h->have_state) return h->fail(DSMC_ERR_STATE, "tile size must be set before the particles are loaded");
+  if (h->have_state || h->h.tile_order)
+    return h->fail(DSMC_ERR_STATE, "tile size must be set before the particles are loaded");
   if (tile_cells < 32u || tile_cells > kTileCells || (tile_cells & (tile_cells - 1u)))
     return h->fail(DSMC_ERR_INVALID, "tile_cells must be a power of two in [32, %u]", kTileCells);
   h->h.tile_cells = tile_cells;
