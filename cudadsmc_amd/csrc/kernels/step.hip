@@ -52,14 +52,19 @@ __device__ __forceinline__ void mt_store(const InstDesc& I, uint32_t s, const Mt
   for (int w = 0; w < kMtN; ++w) gmem(I.mt_state)[(uint32_t)w * I.n_streams + s] = m.st[w];
 }
 
-// Wave-aggregated atomicAdd(&cnt[key], 1) for keys of consecutive lanes,
-// returning each lane's count before its own increment: one atomic per run
+// Wave-aggregated atomicAdd(&cnt[key], 1) for keys of consecutive lanes
+// (cnt global or in LDS), returning each lane's count before its own
+// increment: one atomic per run
 // of equal keys, whose lanes take consecutive values. Invalid keys are
 // skipped (their result is meaningless). Split in two so that a caller can
 // issue the atomics of several rounds before waiting for any: count_issue
 // returns the run head's old count (0 elsewhere) and the wave's run heads,
 // count_finish the lane's value.
-__device__ __forceinline__ uint32_t count_issue(gptr<uint32_t> cnt, uint32_t key, uint64_t& heads) {
+__device__ __forceinline__ uint32_t add_returning(gptr<uint32_t> p, uint32_t v) { return gatomic_add(p, v); }
+__device__ __forceinline__ uint32_t add_returning(uint32_t* p, uint32_t v) { return atomicAdd(p, v); }  // LDS
+
+template <class P>
+__device__ __forceinline__ uint32_t count_issue(P cnt, uint32_t key, uint64_t& heads) {
   const uint32_t lane = lane_id();
   const uint32_t prev = __shfl_up(key, 1);
   const bool head = (lane == 0) || (key != prev);
@@ -69,7 +74,7 @@ __device__ __forceinline__ uint32_t count_issue(gptr<uint32_t> cnt, uint32_t key
     const uint64_t below_incl = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
     const uint64_t above = heads & ~below_incl;
     const uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
-    base = gatomic_add(cnt + key, next - lane);
+    base = add_returning(cnt + key, next - lane);
   }
   return base;
 }
@@ -431,7 +436,9 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __
 // cells; the others (they left their tile, went through the wall pass, or
 // were injected after the tile-ordered slots) take the places from the
 // cursor on, with one wave-aggregated global atomic per run of equal keys
-// and the atomics of kScatterRounds rounds of 256 slots in flight together.
+// and the atomics of kScatterRounds rounds of 256 slots in flight together
+// (one atomic per particle is far slower: runs are common, e.g. a cell's
+// particles after the wall pass).
 // The order within a cell is the order in which the atomics landed; the
 // tile kernel sorts each cell's segment by storage index.
 constexpr uint32_t kScatterRounds = 4;
@@ -452,52 +459,69 @@ __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip
   // particle through the cursors (their tile kernel counts none as stayed)
   const uint32_t S = I.tile_dyn ? min(tp0[I.ntiles], n) : 0u;
   const uint32_t K = kScatterSpan / I.tile_cells;  // tiles per pass
-  // rounds of 256 slots from q0 (below q1); tiles [t0, t0 + K) when K > 0,
-  // whose slot runs start at s_tp[] and whose cells' next places are s_cnt[]
-  auto place = [&](uint32_t q0, uint32_t q1, uint32_t t0, uint32_t K) {
-    const uint32_t c0 = t0 * I.tile_cells;
-    uint32_t key[kScatterRounds], base[kScatterRounds], loc[kScatterRounds];
+  // Slots of the pass's tiles [t0, t0 + K) (runs starting at s_tp[]), in
+  // rounds of 256 slots from q0 (below q1). Each lane keeps the tile of its
+  // last slot in t (its slots only grow). A particle that stayed in its
+  // tile takes the next LDS place of its cell (s_cnt, seeded with the cell
+  // starts); any other takes its cell's cursor, with one wave-aggregated
+  // atomic per run of equal keys (particles of one cell that leave together,
+  // or a cell's particles after the wall pass), the atomics of
+  // kScatterRounds rounds issued before any result is used.
+  auto place_tiled = [&](uint32_t q0, uint32_t q1, uint32_t c0, uint32_t K, uint32_t& t) {
+    uint32_t loc[kScatterRounds], g[kScatterRounds], base[kScatterRounds];
     uint64_t heads[kScatterRounds];
 #pragma unroll
     for (uint32_t r = 0; r < kScatterRounds; ++r) {
       const uint32_t i = q0 + r * 256u + tid;
-      key[r] = i < q1 ? keys[i] : kInvalidKey;
-      // stayed: its cell lies in the tile whose slots hold it
-      loc[r] = kInvalidKey;
-      const uint32_t rel = key[r] - c0;
-      if (K && rel < K * I.tile_cells) {
-        uint32_t t = 0;
-        for (uint32_t h = K; h > 1u;) {  // the last tile whose first slot is <= i
-          const uint32_t half = h >> 1;
-          if (s_tp[t + half] <= i) t += half;
-          h -= half;
-        }
-        if (rel / I.tile_cells == t) loc[r] = rel;
-      }
-      const uint32_t g = loc[r] != kInvalidKey ? kInvalidKey : key_cell(key[r]);
-      base[r] = count_issue(cursor, g, heads[r]);
+      const uint32_t key = i < q1 ? keys[i] : kInvalidKey;
+      while (t + 1u < K && s_tp[t + 1u] <= i) ++t;
+      const uint32_t rel = key - c0;
+      const bool stayed = rel - t * I.tile_cells < I.tile_cells;  // its cell lies in the tile that holds its slot
+      loc[r] = stayed ? rel : kInvalidKey;
+      g[r] = stayed ? kInvalidKey : key_cell(key);
+      base[r] = count_issue(cursor, g[r], heads[r]);
     }
 #pragma unroll
     for (uint32_t r = 0; r < kScatterRounds; ++r) {
       const uint32_t i = q0 + r * 256u + tid;
       uint32_t dst = count_finish(base[r], heads[r]);
       if (loc[r] != kInvalidKey) dst = atomicAdd(&s_cnt[loc[r]], 1u);
-      if (key_cell(key[r]) != kInvalidKey && dst < I.cap) perm[dst] = i;
+      if ((loc[r] != kInvalidKey || g[r] != kInvalidKey) && dst < I.cap) perm[dst] = i;
+    }
+  };
+  // slots after the tile-ordered ones (or all of them before the first
+  // step): through the cursors, one wave-aggregated atomic per run of
+  // equal keys
+  auto place_rest = [&](uint32_t q0, uint32_t q1) {
+    uint32_t key[kScatterRounds], base[kScatterRounds];
+    uint64_t heads[kScatterRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kScatterRounds; ++r) {
+      const uint32_t i = q0 + r * 256u + tid;
+      key[r] = i < q1 ? key_cell(keys[i]) : kInvalidKey;
+      base[r] = count_issue(cursor, key[r], heads[r]);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kScatterRounds; ++r) {
+      const uint32_t i = q0 + r * 256u + tid;
+      const uint32_t dst = count_finish(base[r], heads[r]);
+      if (key[r] != kInvalidKey && dst < I.cap) perm[dst] = i;
     }
   };
   const uint32_t passes = S ? (I.ntiles + K - 1u) / K : 0u;
   for (uint32_t g = blockIdx.x; g < passes; g += gridDim.x) {  // uniform over the workgroup
     const uint32_t t0 = g * K;
-    // the next place of each cell for particles that stayed: from its start
-    for (uint32_t c = tid; c < kScatterSpan; c += 256u) s_cnt[c] = t0 * I.tile_cells + c < I.ncells ? start[t0 * I.tile_cells + c] : 0u;
+    const uint32_t c0 = t0 * I.tile_cells;
+    for (uint32_t c = tid; c < kScatterSpan; c += 256u) s_cnt[c] = c0 + c < I.ncells ? start[c0 + c] : 0u;
     if (tid <= K) s_tp[tid] = t0 + tid <= I.ntiles ? min(tp0[t0 + tid], S) : S;
     __syncthreads();
     const uint32_t p0 = s_tp[0], p1 = max(s_tp[K], p0);
-    for (uint32_t q0 = p0; q0 < p1; q0 += 256u * kScatterRounds) place(q0, p1, t0, K);
+    uint32_t t = 0;
+    for (uint32_t q0 = p0; q0 < p1; q0 += 256u * kScatterRounds) place_tiled(q0, p1, c0, K, t);
     __syncthreads();  // s_cnt and s_tp are set again for the next pass
   }
   for (uint32_t q0 = S + blockIdx.x * 256u * kScatterRounds; q0 < n; q0 += gridDim.x * 256u * kScatterRounds)
-    place(q0, n, 0u, 0u);
+    place_rest(q0, n);
 }
 
 // ============================================================ sort operator
