@@ -125,6 +125,31 @@ def measured_traffic(workload: str, phase: str):
     return None
 
 
+def stored_warm_start():
+    """The second half of the metric, warm- against cold-start time to
+    steady on 32 held-out C5 cases, as measured by the committed evaluation
+    run (training takes minutes, so it is not repeated here); None if
+    absent."""
+    p = ROOT / "profiles" / "warmstart_c5_eval.txt"
+    out = {}
+    try:
+        for line in p.read_text().splitlines():
+            if line.startswith("#") or ":" not in line:
+                continue
+            k, v = (x.strip() for x in line.split(":", 1))
+            out[k] = None if v == "null" else float(v)
+    except (OSError, ValueError):
+        return None
+    keys = ("steps_ratio_median", "steps_ratio_median_dense", "steps_ratio_median_rarefied", "median_steps_cold",
+            "median_steps_warm", "cold_reached", "warm_reached", "median_pred_err", "wall_ratio")
+    res = {k: out.get(k) for k in keys}
+    res.update({"cases": out.get("cases"), "particles_per_case": out.get("N"), "target": ">= 3x",
+                "source": "stored: profiles/warmstart_c5_eval.txt (python -m cudadsmc_amd.warmstart data/train/eval "
+                          "on one MI355X; steady = accumulated block fields within 2% L2 of a long cold "
+                          "reference and C_d within 3 sigma)"})
+    return res
+
+
 def traffic_phase(first_step: int) -> str:
     """Which stored traffic figure fits timed steps that start at `first_step`."""
     return "early" if first_step < 500 else "developed"
@@ -343,6 +368,7 @@ def main() -> int:
                                        f"{traffic_phase(args.warmup + 1)} flow, copy-calibrated)",
                      "kernel": "k_tile (56 B/particle + 20 B/cell algorithmic)"},
         "validate_bad_particles": int(bad),
+        "warm_vs_cold_time_to_steady": stored_warm_start(),
     }
     if not args.no_developed:
         # the same solver continued into the developed flow, then timed again
