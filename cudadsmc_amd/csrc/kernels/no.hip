@@ -239,14 +239,49 @@ __global__ void k_no_reduce_partials(const float* part, long n, int chunks, floa
   out[i] = accumulate ? out[i] + s : s;
 }
 
-// db[co] partial sums over chunks of voxels: part[chunk][co]
-__global__ void k_no_bias_grad(const bf16* dy, long M, int Cout, int chunk, float* part) {
-  const int co = threadIdx.x;
-  if (co >= Cout) return;
-  long m0 = (long)blockIdx.x * chunk;
+// db[co] partial sums: workgroup b sums voxels [b * kBiasChunk, (b + 1) *
+// kBiasChunk) of every channel into part[b][co]; each thread takes channel
+// co = 32 j + (t mod 32) and every eighth voxel (coalesced 64-byte rows),
+// and the eight voxel lanes are added in a fixed order.
+constexpr int kBiasChunk = 512;
+
+__global__ void __launch_bounds__(256) k_no_bias_grad(const bf16* dy, long M, int Cout, float* part) {
+  __shared__ float red[8][kMaxC];
+  const int t = threadIdx.x, cl = t & 31, vl = t >> 5;
+  const long m0 = (long)blockIdx.x * kBiasChunk;
+  for (int j = 0; j < Cout / 32; ++j) {
+    const int co = j * 32 + cl;
+    float s = 0.0f;
+#pragma unroll 8
+    for (int i = vl; i < kBiasChunk; i += 8) {
+      const long m = m0 + i;
+      if (m < M) s += bf2f(dy[m * Cout + co]);
+    }
+    red[vl][co] = s;
+  }
+  __syncthreads();
+  if (t < Cout) {
+    float s = 0.0f;
+    for (int v = 0; v < 8; ++v) s += red[v][t];
+    part[(long)blockIdx.x * Cout + t] = s;
+  }
+}
+
+// out[i] (+)= sum over c of part[c][i] for many partial rows: a workgroup
+// per output, the rows split over its threads and added in a fixed order.
+__global__ void __launch_bounds__(256) k_no_reduce_rows(const float* part, int n, int rows, float* out,
+                                                        int accumulate) {
+  __shared__ float red[256];
+  const int i = blockIdx.x, t = threadIdx.x;
   float s = 0.0f;
-  for (long m = m0; m < m0 + chunk && m < M; ++m) s += bf2f(dy[m * Cout + co]);
-  part[(long)blockIdx.x * Cout + co] = s;
+  for (int c = t; c < rows; c += 256) s += part[(long)c * n + i];
+  red[t] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) out[i] = accumulate ? out[i] + red[0] : red[0];
 }
 
 // dz = dy * (h > 0)  (ReLU backward through the stored activation)
@@ -388,14 +423,14 @@ int dsmc_no_conv_bwd_w(const void* x, const void* dy, float* dw, float* part, lo
 
 int dsmc_no_bias_grad(const void* dy, long M, int Cout, float* db, float* part, long part_floats, int accumulate,
                       void* stream) {
-  const int chunk = 4096;
   NO_EMPTY(M);
-  int chunks = (int)nb(M, chunk);
-  if ((long)chunks * Cout > part_floats || Cout > 1024) return -5;
-  hipLaunchKernelGGL(k_no_bias_grad, dim3(chunks), dim3(((Cout + 63) / 64) * 64), 0, (hipStream_t)stream,
-                     (const bf16*)dy, M, Cout, chunk, part);
-  hipLaunchKernelGGL(k_no_reduce_partials, dim3(nb(Cout, 256)), dim3(256), 0, (hipStream_t)stream, part,
-                     (long)Cout, chunks, db, accumulate);
+  if (Cout % 32 || Cout > kMaxC) return -1;
+  int chunks = (int)nb(M, kBiasChunk);
+  if ((long)chunks * Cout > part_floats) return -5;
+  hipLaunchKernelGGL(k_no_bias_grad, dim3(chunks), dim3(256), 0, (hipStream_t)stream, (const bf16*)dy, M, Cout,
+                     part);
+  hipLaunchKernelGGL(k_no_reduce_rows, dim3(Cout), dim3(256), 0, (hipStream_t)stream, part, Cout, chunks, db,
+                     accumulate);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

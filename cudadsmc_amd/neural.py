@@ -191,7 +191,8 @@ class FieldOperator:
         self.dz = torch.zeros(M, W, **bf)
         chunks = (M + 4095) // 4096
         self.part = torch.zeros(chunks * 27 * W * W, dtype=torch.float32, device="cuda")
-        self.bpart = torch.zeros(max(chunks, 1) * max(W, C_OUT), dtype=torch.float32, device="cuda")
+        # bias-gradient partials: one row per 512 voxels
+        self.bpart = torch.zeros(max((M + 511) // 512, 1) * max(W, C_OUT), dtype=torch.float32, device="cuda")
         self.loss_part = torch.zeros((M * C_OUT + 255) // 256, dtype=torch.float32, device="cuda")
 
     def sync_bf16(self) -> None:

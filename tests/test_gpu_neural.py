@@ -123,7 +123,7 @@ def test_bias_grad_and_relu_bwd():
     M, Cc = 9000, 96
     dy = no_ref.bf16_round(rng.standard_normal((M, Cc)).astype(np.float32))
     h = no_ref.bf16_round(rng.standard_normal((M, Cc)).astype(np.float32))
-    part = torch.zeros(((M + 4095) // 4096) * Cc, dtype=torch.float32, device="cuda")
+    part = torch.zeros(((M + 511) // 512) * Cc, dtype=torch.float32, device="cuda")  # one row per 512 voxels
     db = torch.zeros(Cc, dtype=torch.float32, device="cuda")
     neural.bias_grad(bf(dy), M, Cc, db, part)
     np.testing.assert_allclose(host(db), dy.astype(np.float64).sum(0), rtol=1e-5, atol=1e-3)
