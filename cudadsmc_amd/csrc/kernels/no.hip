@@ -30,7 +30,7 @@ namespace {
 
 constexpr int kWaves = 4;
 constexpr int kMaxNTiles = 4;   // Cout <= 128
-constexpr int kChunk = 64;      // voxels staged per LDS round in conv_bwd_w
+constexpr int kChunk = 64;      // voxels staged per LDS round in conv_bwd_w (one per lane)
 constexpr int kMaxC = 128;
 
 __device__ __forceinline__ void tap_offset(int t, int taps, int dil, int& dz, int& dy, int& dx) {
@@ -171,33 +171,40 @@ __global__ void __launch_bounds__(256) k_no_conv_bwd_w(NoBwdWArgs a) {
   const int cgy = a.Cout / 8, cgx = a.Cin / 8;
   for (long k0 = m0; k0 < m0 + a.chunk && k0 < M; k0 += kChunk) {
     __syncthreads();
-    // stage dy and shifted x of kChunk voxels, transposed to [c][k]
-    for (int q = threadIdx.x; q < kChunk * (cgy + cgx); q += blockDim.x) {
-      const int kk = q / (cgy + cgx);
-      const int g = q - kk * (cgy + cgx);
+    // stage dy and the shifted x of kChunk voxels, transposed to [c][k]:
+    // lane = voxel, so consecutive lanes write consecutive LDS halfwords
+    // (no bank conflicts), and each lane works out its voxel's source once;
+    // the waves share out the 8-channel groups
+    {
+      static_assert(kChunk == 64, "one staged voxel per lane of a wavefront");
+      const int kk = lane;
       const long mm = k0 + kk;
-      bf16x8 v;
+      long src = -1;  // the voxel x is read from for this tap (none outside the grid)
+      if (mm < M) {
+        const int b = (int)(mm / V);
+        int r = (int)(mm - (long)b * V);
+        const int z = r / (a.Y * a.X);
+        r -= z * a.Y * a.X;
+        const int y = r / a.X;
+        const int x = r - y * a.X;
+        const int zz = z + dz, yy = y + dyo, xx = x + dx;
+        if (zz >= 0 && zz < a.Z && yy >= 0 && yy < a.Y && xx >= 0 && xx < a.X)
+          src = (((long)b * a.Z + zz) * a.Y + yy) * a.X + xx;
+      }
+      for (int g = wave; g < cgy + cgx; g += kWaves) {
+        bf16x8 v;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) v[e] = (bf16)0.0f;
-      if (g < cgy) {
-        if (mm < M) v = *reinterpret_cast<const bf16x8*>(a.dy + mm * a.Cout + g * 8);
+        for (int e = 0; e < 8; ++e) v[e] = (bf16)0.0f;
+        if (g < cgy) {
+          if (mm < M) v = *reinterpret_cast<const bf16x8*>(a.dy + mm * a.Cout + g * 8);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) sdy[g * 8 + e][kk] = v[e];
-      } else {
-        const int gx = g - cgy;
-        if (mm < M) {
-          int b = (int)(mm / V);
-          int r = (int)(mm - (long)b * V);
-          int z = r / (a.Y * a.X);
-          r -= z * a.Y * a.X;
-          int y = r / a.X;
-          int x = r - y * a.X;
-          int zz = z + dz, yy = y + dyo, xx = x + dx;
-          if (zz >= 0 && zz < a.Z && yy >= 0 && yy < a.Y && xx >= 0 && xx < a.X)
-            v = *reinterpret_cast<const bf16x8*>(a.x + ((((long)b * a.Z + zz) * a.Y + yy) * a.X + xx) * a.Cin + gx * 8);
+          for (int e = 0; e < 8; ++e) sdy[g * 8 + e][kk] = v[e];
+        } else {
+          const int gx = g - cgy;
+          if (src >= 0) v = *reinterpret_cast<const bf16x8*>(a.x + src * a.Cin + gx * 8);
+#pragma unroll
+          for (int e = 0; e < 8; ++e) sx[gx * 8 + e][kk] = v[e];
         }
-#pragma unroll
-        for (int e = 0; e < 8; ++e) sx[gx * 8 + e][kk] = v[e];
       }
     }
     __syncthreads();
