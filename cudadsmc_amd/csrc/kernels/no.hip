@@ -10,7 +10,8 @@
 //  * conv_fwd: implicit GEMM, M = voxels, N = Cout, K = taps*Cin. Each wave
 //    owns 32 voxels and all Cout (Cout/32 MFMA tiles); A fragments are
 //    16-byte loads of 8 channels of the tap's neighbour voxel (zero outside
-//    the grid), B fragments 16-byte loads of the K-contiguous weights.
+//    the grid), B fragments 16-byte LDS reads of the K-contiguous weights,
+//    which the workgroup stages one tap ahead.
 //    Epilogue: bias, optional ReLU, optional residual add, BF16 or FP32 out.
 //  * the data gradient is conv_fwd on mirrored, transposed weights
 //    (weight_flip), applied to the ReLU-masked output gradient.
@@ -32,6 +33,7 @@ constexpr int kWaves = 4;
 constexpr int kMaxNTiles = 4;   // Cout <= 128
 constexpr int kChunk = 64;      // voxels staged per LDS round in conv_bwd_w (one per lane)
 constexpr int kMaxC = 128;
+constexpr int kWRegs = kMaxC * kMaxC / 8 / 256;  // 16-byte weight pieces per thread when staging a tap
 
 __device__ __forceinline__ void tap_offset(int t, int taps, int dil, int& dz, int& dy, int& dx) {
   if (taps == 1) {
@@ -81,13 +83,44 @@ __global__ void __launch_bounds__(256) k_no_conv_fwd(NoConvArgs a) {
     y = r / a.X;
     x = r - y * a.X;
   }
+  // The workgroup's four waves share each tap's weights: they are staged in
+  // LDS ([Cout][Cin + 8], two buffers), tap t + 1's copied in while tap t is
+  // computed, so the B fragments are LDS reads instead of a per-wave reload
+  // through L1.
+  extern __shared__ __align__(16) bf16 sw[];
+  const int ldw = a.Cin + 8;
+  const int wchunks = a.Cout * (a.Cin >> 3);  // 16-byte pieces of one tap's weights
+  bf16x8 wreg[kWRegs];
+  auto fetch_w = [&](int t) {
+    const bf16* src = a.w + (long)t * a.Cout * a.Cin;
+#pragma unroll
+    for (int r = 0; r < kWRegs; ++r) {
+      const int q = threadIdx.x + r * 256;
+      if (q < wchunks) wreg[r] = *reinterpret_cast<const bf16x8*>(src + (long)q * 8);
+    }
+  };
+  auto store_w = [&](int buf) {
+    bf16* dst = sw + (long)buf * a.Cout * ldw;
+#pragma unroll
+    for (int r = 0; r < kWRegs; ++r) {
+      const int q = threadIdx.x + r * 256;
+      if (q < wchunks) {
+        const int co = q / (a.Cin >> 3), c8 = q - co * (a.Cin >> 3);
+        *reinterpret_cast<bf16x8*>(dst + co * ldw + c8 * 8) = wreg[r];
+      }
+    }
+  };
+  fetch_w(0);
+  store_w(0);
+  __syncthreads();
   for (int t = 0; t < a.taps; ++t) {
+    if (t + 1 < a.taps) fetch_w(t + 1);
     int dz, dy, dx;
     tap_offset(t, a.taps, a.dil, dz, dy, dx);
     const int zz = z + dz, yy = y + dy, xx = x + dx;
     const bool in = valid && zz >= 0 && zz < a.Z && yy >= 0 && yy < a.Y && xx >= 0 && xx < a.X;
     const bf16* xp = a.x + ((((long)b * a.Z + zz) * a.Y + yy) * a.X + xx) * a.Cin;
-    const bf16* wp = a.w + (long)t * a.Cout * a.Cin;
+    const bf16* wl = sw + (long)(t & 1) * a.Cout * ldw;
     for (int c0 = 0; c0 < a.Cin; c0 += 16) {
       bf16x8 av;
       if (in) {
@@ -100,11 +133,13 @@ __global__ void __launch_bounds__(256) k_no_conv_fwd(NoConvArgs a) {
       for (int j = 0; j < kMaxNTiles; ++j) {
         if (j < ntiles) {
           const int co = j * 32 + (lane & 31);
-          bf16x8 bv = *reinterpret_cast<const bf16x8*>(wp + (long)co * a.Cin + c0 + kg * 8);
+          const bf16x8 bv = *reinterpret_cast<const bf16x8*>(wl + co * ldw + c0 + kg * 8);
           acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[j], 0, 0, 0);
         }
       }
     }
+    if (t + 1 < a.taps) store_w((t + 1) & 1);  // that buffer was last read in tap t - 1
+    __syncthreads();
   }
   // epilogue: lane holds D[row][col], col = lane % 32,
   // row = 8 * (i / 4) + 4 * (lane / 32) + i % 4
@@ -394,12 +429,14 @@ extern "C" {
 int dsmc_no_conv_fwd(const void* x, const void* w, const float* bias, const void* res, void* y, float* y32,
                      void* h, int B, int Z, int Y, int X, int Cin, int Cout, int taps, int dil, int relu,
                      void* stream) {
-  if (Cin % 16 || Cout % 32 || Cout > 32 * kMaxNTiles || (taps != 1 && taps != 27) || B <= 0) return -1;
+  if (Cin % 16 || Cin > kMaxC || Cout % 32 || Cout > 32 * kMaxNTiles || (taps != 1 && taps != 27) || B <= 0)
+    return -1;
   NoConvArgs a{(const bf16*)x, (const bf16*)w, bias, (const bf16*)res, (bf16*)y, y32, (bf16*)h,
                B, Z, Y, X, Cin, Cout, taps, dil, relu};
   long M = (long)B * Z * Y * X;
   NO_EMPTY(M);
-  hipLaunchKernelGGL(k_no_conv_fwd, dim3(nb(M, 32 * kWaves)), dim3(256), 0, (hipStream_t)stream, a);
+  const size_t lds = 2ull * Cout * (Cin + 8) * sizeof(bf16);  // two taps of weights
+  hipLaunchKernelGGL(k_no_conv_fwd, dim3(nb(M, 32 * kWaves)), dim3(256), lds, (hipStream_t)stream, a);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
