@@ -25,13 +25,15 @@
 
 typedef __bf16 bf16;
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(2 * sizeof(__bf16)))) __bf16 bf16x2;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 namespace {
 
 constexpr int kWaves = 4;
 constexpr int kMaxNTiles = 4;   // Cout <= 128
-constexpr int kChunk = 64;      // voxels staged per LDS round in conv_bwd_w (one per lane)
+constexpr int kChunk = 128;     // voxels staged per LDS round in conv_bwd_w (two per lane)
+constexpr int kLdk = kChunk + 8;  // LDS row length of the staged operands
 constexpr int kMaxC = 128;
 constexpr int kWRegs = kMaxC * kMaxC / 8 / 256;  // 16-byte weight pieces per thread when staging a tap
 
@@ -185,8 +187,11 @@ struct NoBwdWArgs {
 };
 
 __global__ void __launch_bounds__(256) k_no_conv_bwd_w(NoBwdWArgs a) {
-  __shared__ __align__(16) bf16 sdy[kMaxC][kChunk + 8];
-  __shared__ __align__(16) bf16 sx[kMaxC][kChunk + 8];
+  // dy and the shifted x of kChunk voxels, transposed: [Cout][kLdk] then
+  // [Cin][kLdk] (dynamic LDS sized by the call's channel counts)
+  extern __shared__ __align__(16) bf16 sbw[];
+  bf16* const sdy = sbw;
+  bf16* const sx = sbw + (long)a.Cout * kLdk;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int kg = lane >> 5;
@@ -204,41 +209,52 @@ __global__ void __launch_bounds__(256) k_no_conv_bwd_w(NoBwdWArgs a) {
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[j][i] = 0.0f;
   const int cgy = a.Cout / 8, cgx = a.Cin / 8;
-  for (long k0 = m0; k0 < m0 + a.chunk && k0 < M; k0 += kChunk) {
+  const long m_end = m0 + a.chunk < M ? m0 + a.chunk : M;  // this workgroup's voxels: [m0, m_end)
+  for (long k0 = m0; k0 < m_end; k0 += kChunk) {
     __syncthreads();
     // stage dy and the shifted x of kChunk voxels, transposed to [c][k]:
-    // lane = voxel, so consecutive lanes write consecutive LDS halfwords
-    // (no bank conflicts), and each lane works out its voxel's source once;
-    // the waves share out the 8-channel groups
+    // lane l takes voxels 2l and 2l + 1, so a channel's two values form one
+    // 32-bit LDS word and consecutive lanes write consecutive words (no bank
+    // conflicts); each lane works out its voxels' sources once, and the
+    // waves share out the 8-channel groups
     {
-      static_assert(kChunk == 64, "one staged voxel per lane of a wavefront");
-      const int kk = lane;
-      const long mm = k0 + kk;
-      long src = -1;  // the voxel x is read from for this tap (none outside the grid)
-      if (mm < M) {
-        const int b = (int)(mm / V);
-        int r = (int)(mm - (long)b * V);
-        const int z = r / (a.Y * a.X);
-        r -= z * a.Y * a.X;
-        const int y = r / a.X;
-        const int x = r - y * a.X;
-        const int zz = z + dz, yy = y + dyo, xx = x + dx;
-        if (zz >= 0 && zz < a.Z && yy >= 0 && yy < a.Y && xx >= 0 && xx < a.X)
-          src = (((long)b * a.Z + zz) * a.Y + yy) * a.X + xx;
+      static_assert(kChunk == 128, "two staged voxels per lane of a wavefront");
+      long mm[2], src[2];  // the voxels, and the voxels x is read from for this tap (-1: outside)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        mm[h] = k0 + 2 * lane + h;
+        src[h] = -1;
+        if (mm[h] < m_end) {
+          const int b = (int)(mm[h] / V);
+          int r = (int)(mm[h] - (long)b * V);
+          const int z = r / (a.Y * a.X);
+          r -= z * a.Y * a.X;
+          const int y = r / a.X;
+          const int x = r - y * a.X;
+          const int zz = z + dz, yy = y + dyo, xx = x + dx;
+          if (zz >= 0 && zz < a.Z && yy >= 0 && yy < a.Y && xx >= 0 && xx < a.X)
+            src[h] = (((long)b * a.Z + zz) * a.Y + yy) * a.X + xx;
+        }
       }
       for (int g = wave; g < cgy + cgx; g += kWaves) {
-        bf16x8 v;
+        bf16x8 v[2];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = (bf16)0.0f;
-        if (g < cgy) {
-          if (mm < M) v = *reinterpret_cast<const bf16x8*>(a.dy + mm * a.Cout + g * 8);
+        for (int h = 0; h < 2; ++h) {
 #pragma unroll
-          for (int e = 0; e < 8; ++e) sdy[g * 8 + e][kk] = v[e];
-        } else {
-          const int gx = g - cgy;
-          if (src >= 0) v = *reinterpret_cast<const bf16x8*>(a.x + src * a.Cin + gx * 8);
+          for (int e = 0; e < 8; ++e) v[h][e] = (bf16)0.0f;
+          if (g < cgy) {
+            if (mm[h] < m_end) v[h] = *reinterpret_cast<const bf16x8*>(a.dy + mm[h] * a.Cout + g * 8);
+          } else if (src[h] >= 0) {
+            v[h] = *reinterpret_cast<const bf16x8*>(a.x + src[h] * a.Cin + (g - cgy) * 8);
+          }
+        }
+        bf16* row = g < cgy ? sdy + (long)(g * 8) * kLdk : sx + (long)((g - cgy) * 8) * kLdk;
 #pragma unroll
-          for (int e = 0; e < 8; ++e) sx[gx * 8 + e][kk] = v[e];
+        for (int e = 0; e < 8; ++e) {
+          bf16x2 pr;
+          pr[0] = v[0][e];
+          pr[1] = v[1][e];
+          *reinterpret_cast<bf16x2*>(row + (long)e * kLdk + 2 * lane) = pr;
         }
       }
     }
@@ -252,8 +268,8 @@ __global__ void __launch_bounds__(256) k_no_conv_bwd_w(NoBwdWArgs a) {
       const int ci = (tile % tn) * 32 + (lane & 31);
 #pragma unroll
       for (int kc = 0; kc < kChunk; kc += 16) {
-        bf16x8 av = *reinterpret_cast<const bf16x8*>(&sdy[co][kc + kg * 8]);
-        bf16x8 bv = *reinterpret_cast<const bf16x8*>(&sx[ci][kc + kg * 8]);
+        bf16x8 av = *reinterpret_cast<const bf16x8*>(sdy + (long)co * kLdk + kc + kg * 8);
+        bf16x8 bv = *reinterpret_cast<const bf16x8*>(sx + (long)ci * kLdk + kc + kg * 8);
         acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[j], 0, 0, 0);
       }
     }
@@ -450,7 +466,7 @@ int dsmc_no_weight_flip(const void* w, void* wb, int taps, int Cout, int Cin, vo
 
 int dsmc_no_conv_bwd_w(const void* x, const void* dy, float* dw, float* part, long part_floats, int B, int Z,
                        int Y, int X, int Cin, int Cout, int taps, int dil, int chunk, int accumulate, void* stream) {
-  if (Cin % 32 || Cout % 32 || Cin > kMaxC || Cout > kMaxC || chunk % kChunk || (taps != 1 && taps != 27))
+  if (Cin % 32 || Cout % 32 || Cin > kMaxC || Cout > kMaxC || chunk <= 0 || chunk % 64 || (taps != 1 && taps != 27))
     return -1;
   if ((Cout / 32) * (Cin / 32) > kWaves * kMaxNTiles) return -1;
   long M = (long)B * Z * Y * X;
@@ -459,7 +475,8 @@ int dsmc_no_conv_bwd_w(const void* x, const void* dy, float* dw, float* part, lo
   long n = (long)taps * Cout * Cin;
   if ((long)chunks * n > part_floats) return -5;
   NoBwdWArgs a{(const bf16*)x, (const bf16*)dy, part, B, Z, Y, X, Cin, Cout, taps, dil, chunk};
-  hipLaunchKernelGGL(k_no_conv_bwd_w, dim3(chunks, taps), dim3(256), 0, (hipStream_t)stream, a);
+  const size_t lds = (size_t)(Cout + Cin) * kLdk * sizeof(bf16);
+  hipLaunchKernelGGL(k_no_conv_bwd_w, dim3(chunks, taps), dim3(256), lds, (hipStream_t)stream, a);
   hipLaunchKernelGGL(k_no_reduce_partials, dim3(nb(n, 256)), dim3(256), 0, (hipStream_t)stream, part, n, chunks,
                      dw, accumulate);
   return hipGetLastError() == hipSuccess ? 0 : -3;
