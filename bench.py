@@ -109,6 +109,37 @@ def other_workload(name: str, warmup: int, steps: int) -> dict:
     return out
 
 
+def reference_case():
+    """The reference's own benchmark case (test.bat:1, `-f data/config_500k.xml`;
+    kernels/dsmc.cu:398-405: ares.raw, CreateGrid(5,5,20), MoveModel(0,0,-3))
+    in REFERENCE mode with the MT streams (`-mt`), dt = 0.25e-4 s: a solver
+    with the model set, no particles loaded, and the moved mesh."""
+    import cudadsmc_amd as dm
+
+    g = ROOT / "tests" / "golden"  # the reference's data files, kept as test fixtures
+    s = dm.settings_from_file(str(g / "config_500k.xml"))
+    s.benchmark, s.geometry, s.mt = 1, 1, 1
+    s.dt = 0.25e-4
+    sol = dm.Solver()
+    sol.Init(s)
+    mesh = dm.load_raw(str(g / "ares.raw"))
+    sol.SetModel(mesh, grid=(5, 5, 20), move=(0.0, 0.0, -3.0))
+    return sol, s, mesh.translate((0.0, 0.0, -3.0))
+
+
+def reference_benchmark(warmup: int = 200, steps: int = 1000) -> dict:
+    sol, s, mesh = reference_case()
+    n0 = sol.InitSystemState(tuple(s.stream_velocity))
+    sol.DoBenchmark(warmup, s.dt)
+    sol.synchronize()
+    out = throughput_line(sol, s, steps, s.dt)
+    out.update({"particles_loaded": n0, "warmup": warmup, "grid": list(s.grid_dim),
+                "triangles": int(len(mesh.triangles)), "mode": "reference, MT streams, samples every 100th step",
+                "case": "config_500k.xml + ares.raw (test.bat:1, dsmc.cu:398-405)"})
+    sol.close()
+    return out
+
+
 def measured_traffic(workload: str, phase: str):
     """HBM bytes per k_tile launch of this workload in the early ("early":
     steps shortly after the freestream start) or developed ("developed":
@@ -160,10 +191,15 @@ def _oracle_case(name: str):
     GPU (derived scalars, collision grid, RNG layout; no particles there)."""
     import oracle
 
-    sol, s, mesh = make_solver_for(name)
-    o = oracle.OracleSim.from_solver(sol)
-    o.set_geometry(mesh, sol.collision_grid(), sol.fluid_fraction())
-    o.set_near_cells(sol.near_cells())
+    if name == "reference":
+        sol, s, mesh = reference_case()
+        o = oracle.OracleSim.from_solver(sol)
+        o.set_geometry(mesh, sol.collision_grid(), None)
+    else:
+        sol, s, mesh = make_solver_for(name)
+        o = oracle.OracleSim.from_solver(sol)
+        o.set_geometry(mesh, sol.collision_grid(), sol.fluid_fraction())
+        o.set_near_cells(sol.near_cells())
     o.set_rng(sol.rng_state())
     o.set_ntc(sol.ntc_state())
     sol.close()
@@ -394,7 +430,8 @@ def main() -> int:
         print(f"[bench] ensemble {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     if not args.no_other_workloads and args.workload == "c3":
         t0 = time.perf_counter()
-        line["other_workloads"] = {"c1": other_workload("c1", 1000, 1000), "c2": other_workload("c2", 1000, 500)}
+        line["other_workloads"] = {"c1": other_workload("c1", 1000, 1000), "c2": other_workload("c2", 1000, 500),
+                                   "reference_benchmark": reference_benchmark()}
         print(f"[bench] c1/c2 {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
@@ -405,6 +442,8 @@ def main() -> int:
                 "c1_full": cpu_baseline("c1", 600.0, 2000, sampling_from=1000),
                 "c1_one_thread": cpu_baseline("c1", 10.0, 2000, threads=1),
                 "c3_one_thread": cpu_baseline("c3", 10.0, 50, threads=1),
+                # the reference's own benchmark case, REFERENCE mode
+                "reference_benchmark": cpu_baseline("reference", 10.0, 1200),
             }
         print(f"[bench] cpu baseline {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     print(json.dumps(line))

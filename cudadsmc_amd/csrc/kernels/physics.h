@@ -231,6 +231,129 @@ __device__ __forceinline__ uint32_t move_reference(const InstDesc& I, float& px,
   return hits;
 }
 
+// The same flight for the 64 lanes of a wavefront together (lanes without a
+// particle pass act = false). Each call of CollideWithGeometry on a cell
+// searches that cell's triangle list for its first hit in list order; here
+// the lists of all lanes that search in the same call form one pool, each
+// lane tests one entry of it per round, and a lane's first hit is the
+// smallest entry index among its hits (an LDS atomic minimum over the
+// wavefront's `first` array, 64 entries). The hit is then recomputed and
+// applied by its lane alone, exactly as the one-lane loop would.
+__device__ __forceinline__ bool ref_collide_wave(const InstDesc& I, bool want, float& px, float& py, float& pz,
+                                                 float& vx, float& vy, float& vz, float& ox, float& oy, float& oz,
+                                                 uint32_t cell, uint32_t* first) {
+  const uint32_t lane = __lane_id();
+  uint32_t s = 0u, n = 0u;
+  float dx = vx, dy = vy, dz = vz;
+  normalize3(dx, dy, dz);
+  const float ddx = px - ox, ddy = py - oy, ddz = pz - oz;
+  const float dist = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);  // distance4 (vectors.cu:101-106)
+  if (want && !(dist < kGeomEps)) {
+    s = I.cg_start[cell];
+    n = I.cg_end[cell] - s;
+  }
+  uint32_t incl = n;
+#pragma unroll
+  for (uint32_t o = 1; o < 64u; o <<= 1) {
+    const uint32_t x = __shfl_up(incl, o);
+    if (lane >= o) incl += x;
+  }
+  const uint32_t total = __shfl(incl, 63);
+  if (total == 0u) return false;  // uniform over the wavefront
+  const uint32_t off = incl - n;
+  if (n) first[lane] = 0xFFFFFFFFu;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t k0 = 0; k0 < total; k0 += 64u) {
+    const uint32_t k = k0 + lane < total ? k0 + lane : total - 1u;
+    uint32_t o = 0;  // owner: the last lane whose first entry number is <= k
+#pragma unroll
+    for (uint32_t step = 32; step; step >>= 1) {
+      const uint32_t m = o + step;
+      if (m < 64u && (uint32_t)__shfl(off, (int)(m & 63u)) <= k) o = m;
+    }
+    const uint32_t e = __shfl(s, (int)o) + (k - __shfl(off, (int)o));
+    const float qx = __shfl(ox, (int)o), qy = __shfl(oy, (int)o), qz = __shfl(oz, (int)o);
+    const float ex = __shfl(dx, (int)o), ey = __shfl(dy, (int)o), ez = __shfl(dz, (int)o);
+    const float qd = __shfl(dist, (int)o);
+    if (k0 + lane < total) {
+      const uint32_t tri = I.cg_tri[e];
+      float t, u, v;
+      if (intersect_tri(qx, qy, qz, ex, ey, ez, I.tv0[tri], I.tv1[tri], I.tv2[tri], t, u, v) && t >= 0.0f &&
+          t <= qd)
+        atomicMin(&first[o], e);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  if (!n) return false;
+  const uint32_t e = first[lane];
+  if (e == 0xFFFFFFFFu) return false;
+  const uint32_t tri = I.cg_tri[e];
+  const float4 a = I.tv0[tri], b = I.tv1[tri], c = I.tv2[tri];
+  float t, u, v;
+  intersect_tri(ox, oy, oz, dx, dy, dz, a, b, c, t, u, v);
+  const float uv = 1.0f - u - v;
+  const float4 nrm = I.tn[tri];
+  const float wx = nrm.x * kModelWidth, wy = nrm.y * kModelWidth, wz = nrm.z * kModelWidth;
+  const float hx = a.x * uv + b.x * u + c.x * v;
+  const float hy = a.y * uv + b.y * u + c.y * v;
+  const float hz = a.z * uv + b.z * u + c.z * v;
+  const float dtn = vx * nrm.x + vy * nrm.y + vz * nrm.z;
+  vx = vx - 2.0f * dtn * nrm.x;
+  vy = vy - 2.0f * dtn * nrm.y;
+  vz = vz - 2.0f * dtn * nrm.z;
+  float rx = vx, ry = vy, rz = vz;
+  normalize3(rx, ry, rz);
+  const float od = dist - t;
+  px = hx + rx * od;
+  py = hy + ry * od;
+  pz = hz + rz * od;
+  ox = hx + wx;
+  oy = hy + wy;
+  oz = hz + wz;
+  return true;
+}
+
+// move_reference for the 64 lanes of a wavefront together (see
+// ref_collide_wave); lanes without a particle pass act = false.
+__device__ __forceinline__ uint32_t move_reference_wave(const InstDesc& I, bool act, float& px, float& py,
+                                                        float& pz, float& vx, float& vy, float& vz, float dt,
+                                                        uint32_t* first) {
+  float ox = px, oy = py, oz = pz;
+  float nx = ox + vx * dt, ny = oy + vy * dt, nz = oz + vz * dt;
+  uint32_t hits = 0;
+  if (I.has_geom) {
+    bool coll = false;  // not reset inside the loop, as in the reference
+    int i = 0;
+    bool live = act;
+    while (__ballot(live) != 0ull) {
+      uint32_t st = kInvalidKey, en = kInvalidKey;
+      if (live) {
+        st = cg_cell(I, ox, oy, oz);
+        en = cg_cell(I, nx, ny, nz);
+      }
+      const bool w1 = live && st != kInvalidKey && I.cg_flag[st] != 0u;
+      const bool c1 = ref_collide_wave(I, w1, nx, ny, nz, vx, vy, vz, ox, oy, oz, st, first);
+      if (w1) coll = c1;
+      const bool w2 = live && !coll && st != en && en != kInvalidKey && I.cg_flag[en] != 0u;
+      const bool c2 = ref_collide_wave(I, w2, nx, ny, nz, vx, vy, vz, ox, oy, oz, en, first);
+      if (w2) coll = c2;
+      if (live) {
+        if (coll) ++hits;
+        live = coll && i++ < kCollisionLimit;
+      }
+    }
+  }
+  if (act) {
+    ref_box(I, nx, ny, nz, vx, vy, vz);
+    px = nx;
+    py = ny;
+    pz = nz;
+  }
+  return hits;
+}
+
 // ---------------------------------------------------------------- physics
 // Nearest front-facing triangle hit along o + s*d, 0 <= s <= dist, over the
 // collision-grid cells the segment crosses (3-D DDA). Ties on s go to the

@@ -938,13 +938,25 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
     if (r0 + k >= rounds) break;  // uniform over the workgroup
     uint32_t q = (r0 + k) * kTileThreads + tid;
     uint32_t key = kInvalidKey;
+    // REFERENCE: the wall flights of the wavefront run together (every lane
+    // takes part, with or without a particle)
+    float rx = 0.0f, ry = 0.0f, rz = 0.0f, rux = 0.0f, ruy = 0.0f, ruz = 0.0f;
+    uint32_t rhits = 0;
+    if (MODE == kModeReference) {
+      if (q < np) {
+        rx = bx[k], ry = by[k], rz = bz[k];
+        rux = vx[q], ruy = vy[q], ruz = vz[q];
+      }
+      rhits = move_reference_wave(I, q < np, rx, ry, rz, rux, ruy, ruz, I.dt, ntl.rmax + (tid & ~63u));
+    }
     if (q < np) {
       const uint32_t src = bsrc[k];
       float x = bx[k], y = by[k], z = bz[k];
       float ux = vx[q], uy = vy[q], uz = vz[q];
       uint32_t hits = 0;
       if (MODE == kModeReference) {
-        hits = move_reference(I, x, y, z, ux, uy, uz, I.dt);
+        x = rx, y = ry, z = rz, ux = rux, uy = ruy, uz = ruz;
+        hits = rhits;
         key = cell_key(I, x, y, z);
       } else if (phys_near(I, s_nearc[own[q]], ux, uy, uz, I.dt)) {
         key = kDeferredKey;  // stored as is; k_walls moves it and counts its key
