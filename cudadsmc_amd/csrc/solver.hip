@@ -39,11 +39,35 @@ const char* global_error() { return g_err.c_str(); }
 
 using namespace dsmc_host;
 
+// A second stream (default priority) on which the freestream injection runs
+// beside the wall pass: both only add to the cell counts, injection writes
+// the slots after the sorted ones and the wall pass deferred slots below
+// them, so they may overlap; the main stream waits for the injection before
+// the next step. Under graph capture the side stream joins the capture
+// through the fork event.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  hipError_t create() {
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&join, hipEventDisableTiming);
+    return e;
+  }
+  ~SideStream() {
+    if (s) (void)hipStreamSynchronize(s);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
 struct dsmc_solver {
   std::string err;
   dsmc_settings s{};
   bool inited = false, have_state = false;
   hipStream_t stream = nullptr;
+  SideStream side;
   InstDesc h{};
   InstDesc* d_desc = nullptr;
   std::vector<void*> bufs;      // lifetime of the handle
@@ -203,6 +227,7 @@ int dsmc_create(int solver_type, dsmc_solver** out) {
     return set_global_error(DSMC_ERR_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e));
   auto* h = new dsmc_solver();
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = h->side.create();
   if (e != hipSuccess) {
     delete h;
     return set_global_error(DSMC_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -1554,13 +1579,24 @@ int enqueue_step(dsmc_solver* h, uint32_t par, uint32_t flags, hipEvent_t* ev) {
   if (ev) HIPC(hipEventRecord(ev[2], st));
   dsmc_launch::tile(I.mode, h->d_desc, par, flags, I.n_vwg, st);
   if (ev) HIPC(hipEventRecord(ev[3], st));
+  // injection beside the wall pass (a phase timer then sees only the part
+  // of the injection that outlasts the wall pass)
+  const bool overlap = I.defer_bits && I.n_face_cells;
+  auto inject = [&](hipStream_t s) {
+    dsmc_launch::inject_count(h->d_desc, std::min(I.n_streams, I.n_face_cells), s);
+    dsmc_launch::step_scan(h->d_desc, 1, 1, I.n_face_cells, h->scan_scratch, s);
+    dsmc_launch::inject_generate(h->d_desc, par ^ 1u, std::min(I.n_streams, I.n_face_cells), s);
+  };
+  if (overlap) {
+    HIPC(hipEventRecord(h->side.fork, st));
+    HIPC(hipStreamWaitEvent(h->side.s, h->side.fork, 0));
+    inject(h->side.s);
+    HIPC(hipEventRecord(h->side.join, h->side.s));
+  }
   if (I.defer_bits) dsmc_launch::walls(h->d_desc, par ^ 1u, flags, I.n_wall_wg, st);
   if (ev) HIPC(hipEventRecord(ev[4], st));
-  if (I.n_face_cells) {
-    dsmc_launch::inject_count(h->d_desc, std::min(I.n_streams, I.n_face_cells), st);
-    dsmc_launch::step_scan(h->d_desc, 1, 1, I.n_face_cells, h->scan_scratch, st);
-    dsmc_launch::inject_generate(h->d_desc, par ^ 1u, std::min(I.n_streams, I.n_face_cells), st);
-  }
+  if (overlap) HIPC(hipStreamWaitEvent(st, h->side.join, 0));
+  else if (I.n_face_cells) inject(st);
   if (ev) HIPC(hipEventRecord(ev[5], st));
   return DSMC_OK;
 }
@@ -1623,6 +1659,7 @@ struct dsmc_batch {
   std::vector<dsmc_solver*> hs;
   std::string err;
   hipStream_t stream = nullptr;
+  SideStream side;
   hipEvent_t ev = nullptr;
   InstDesc* d_descs = nullptr;
   uint32_t* scratch = nullptr;
@@ -1657,12 +1694,24 @@ int enqueue_batch_step(dsmc_batch* b, uint32_t par, uint32_t flags) {
   dsmc_launch::step_scan(b->d_descs, b->n, 0, b->max_cells, b->scratch, st);
   dsmc_launch::scatter(b->d_descs, par, b->scatter_grid, st, b->n);
   dsmc_launch::tile(b->mode, b->d_descs, par, flags, b->max_vwg, st, b->n);
-  if (b->wall_grid) dsmc_launch::walls(b->d_descs, par ^ 1u, flags, b->wall_grid, st, b->n);
-  if (b->max_faces) {
-    dsmc_launch::inject_count(b->d_descs, b->max_inj_threads, st, b->n);
-    dsmc_launch::step_scan(b->d_descs, b->n, 1, b->max_faces, b->scratch, st);
-    dsmc_launch::inject_generate(b->d_descs, par ^ 1u, b->max_inj_threads, st, b->n);
+  // injection beside the wall pass, as for one solver
+  const bool overlap = b->wall_grid && b->max_faces;
+  auto inject = [&](hipStream_t s) {
+    dsmc_launch::inject_count(b->d_descs, b->max_inj_threads, s, b->n);
+    dsmc_launch::step_scan(b->d_descs, b->n, 1, b->max_faces, b->scratch, s);
+    dsmc_launch::inject_generate(b->d_descs, par ^ 1u, b->max_inj_threads, s, b->n);
+  };
+  hipError_t e = hipSuccess;
+  if (overlap) {
+    e = hipEventRecord(b->side.fork, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->side.s, b->side.fork, 0);
+    if (e == hipSuccess) inject(b->side.s);
+    if (e == hipSuccess) e = hipEventRecord(b->side.join, b->side.s);
   }
+  if (b->wall_grid) dsmc_launch::walls(b->d_descs, par ^ 1u, flags, b->wall_grid, st, b->n);
+  if (overlap && e == hipSuccess) e = hipStreamWaitEvent(st, b->side.join, 0);
+  else if (!overlap && b->max_faces) inject(st);
+  if (e != hipSuccess) return b->fail(DSMC_ERR_DEVICE, std::string("batch side stream: ") + hipGetErrorString(e));
   return DSMC_OK;
 }
 
@@ -1696,6 +1745,7 @@ int dsmc_batch_create(dsmc_solver* const* hs, uint32_t n, dsmc_batch** out) {
     if (h->h.defer_bits) b->wall_grid = std::max(b->wall_grid, h->h.n_wall_wg);
   }
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = b->side.create();
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&b->d_descs, sizeof(InstDesc) * n);
   if (e == hipSuccess) e = hipMalloc(&b->scratch, sizeof(uint32_t) * n * (dsmc_launch::kScanMaxTiles + 1));
