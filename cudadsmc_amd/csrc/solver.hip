@@ -327,16 +327,18 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
   }
   // Cells per tile: keep the expected particles of a tile near half the LDS
   // capacity so that few tiles take the global-memory path; on small grids,
-  // halve the tiles (down to 64 cells) until there are 1024 of them, one per
-  // resident workgroup (c1, 64x32x32: 0.123 ms per step with 64-cell tiles,
-  // 0.132 with 256, 0.149 with 32).
+  // halve the tiles (down to 64 cells) until there are 4096 of them, about
+  // four per resident workgroup (c1, 64x32x32, 100k particles: 0.123 ms per
+  // step with 64-cell tiles, 0.132 with 256, 0.149 with 32; c2, 128x64x64,
+  // 1M particles: 0.155 ms with 128-cell tiles, 0.171 with 256, 0.182 with
+  // 64).
   {
     double ppc_exp = phys ? (s.target_particles ? (double)s.target_particles / C : (double)s.density *
                              I.cell_volume / (double)s.fnum)
                           : (double)h->ppc;
     uint32_t tc = kTileCells;
     while (tc > 32u && tc * ppc_exp > 0.5 * kLdsCap) tc >>= 1;
-    while (tc > 64u && (C + tc - 1) / tc < 1024u) tc >>= 1;
+    while (tc > 64u && (C + tc - 1) / tc < 4096u) tc >>= 1;
     I.tile_cells = tc;
     I.ntiles = (C + tc - 1) / tc;
     h->ppc_exp = ppc_exp;
