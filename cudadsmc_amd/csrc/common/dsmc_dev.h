@@ -224,24 +224,10 @@ __device__ __forceinline__ uint32_t gatomic_add(gptr<uint32_t> p, uint32_t v) {
 // every XCD gets patches from all over the domain (the same share of
 // expensive near-body tiles). Otherwise workgroup vw takes positions vw,
 // vw + G, ... Together with the stream of each lane (vw * 256 + lane) the
-// walk fixes the RNG assignment.
+// walk fixes the RNG assignment. The static walk is tile_queue_pos below
+// with e = vw / 8 + k * G / 8 and q = vw mod 8 (G a multiple of 8), or
+// e = vw + k * G otherwise; positions >= ntiles end the walk.
 constexpr uint32_t kTilePatch = 128;
-
-struct TileWalk {
-  uint32_t e, step, xcd;
-  bool xcd_walk;
-};
-
-__host__ __device__ inline TileWalk tile_walk_begin(uint32_t G, uint32_t vw) {
-  if (G % 8u == 0u) return {vw / 8u, G / 8u, vw % 8u, true};
-  return {vw, G, 0u, false};
-}
-
-// Position of the walk's current element (may be >= ntiles: the walk ends).
-__host__ __device__ inline uint32_t tile_walk_pos(const TileWalk& w) {
-  if (!w.xcd_walk) return w.e;
-  return ((w.e / kTilePatch) * 8u + w.xcd) * kTilePatch + w.e % kTilePatch;
-}
 
 // Tile queues of the tile kernel: workgroups take tiles dynamically, one
 // atomic increment per tile. With a workgroup count that is a multiple of 8
