@@ -455,6 +455,7 @@ __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip
   const gptr<const uint32_t> tp0 = gmem<const uint32_t>(I.tile_p0);
   const gptr<uint32_t> perm = gmem(I.perm);
   const gptr<uint32_t> cursor = gmem(I.cursor);
+  const uint32_t cap = I.cap, tile_cells = I.tile_cells;
   // end of the tile-ordered slots; instances with small tiles place every
   // particle through the cursors (their tile kernel counts none as stayed)
   const uint32_t S = I.tile_dyn ? min(tp0[I.ntiles], n) : 0u;
@@ -466,27 +467,41 @@ __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip
   // starts); any other takes its cell's cursor, with one wave-aggregated
   // atomic per run of equal keys (particles of one cell that leave together,
   // or a cell's particles after the wall pass), the atomics of
-  // kScatterRounds rounds issued before any result is used.
-  auto place_tiled = [&](uint32_t q0, uint32_t q1, uint32_t c0, uint32_t K, uint32_t& t) {
-    uint32_t loc[kScatterRounds], g[kScatterRounds], base[kScatterRounds];
-    uint64_t heads[kScatterRounds];
+  // kScatterRounds rounds issued before any result is used. kk holds the
+  // keys of slots q0...; the keys of the next kScatterRounds rounds are
+  // requested before these atomics, so both latencies overlap.
+  auto load_keys = [&](uint32_t q0, uint32_t q1, uint32_t (&kk)[kScatterRounds]) {
 #pragma unroll
     for (uint32_t r = 0; r < kScatterRounds; ++r) {
       const uint32_t i = q0 + r * 256u + tid;
-      const uint32_t key = i < q1 ? keys[i] : kInvalidKey;
+      kk[r] = i < q1 ? keys[i] : kInvalidKey;
+    }
+  };
+  auto place_tiled = [&](uint32_t q0, uint32_t q1, uint32_t c0, uint32_t K, uint32_t& t,
+                         uint32_t (&kk)[kScatterRounds]) {
+    uint32_t kc[kScatterRounds], loc[kScatterRounds], g[kScatterRounds], base[kScatterRounds], lds[kScatterRounds];
+    uint64_t heads[kScatterRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kScatterRounds; ++r) kc[r] = kk[r];
+    if (q0 + 256u * kScatterRounds < q1) load_keys(q0 + 256u * kScatterRounds, q1, kk);  // uniform
+#pragma unroll
+    for (uint32_t r = 0; r < kScatterRounds; ++r) {
+      const uint32_t i = q0 + r * 256u + tid;
+      const uint32_t key = kc[r];
       while (t + 1u < K && s_tp[t + 1u] <= i) ++t;
       const uint32_t rel = key - c0;
-      const bool stayed = rel - t * I.tile_cells < I.tile_cells;  // its cell lies in the tile that holds its slot
+      const bool stayed = rel - t * tile_cells < tile_cells;  // its cell lies in the tile that holds its slot
       loc[r] = stayed ? rel : kInvalidKey;
       g[r] = stayed ? kInvalidKey : key_cell(key);
       base[r] = count_issue(cursor, g[r], heads[r]);
+      if (stayed) lds[r] = atomicAdd(&s_cnt[rel], 1u);
     }
 #pragma unroll
     for (uint32_t r = 0; r < kScatterRounds; ++r) {
       const uint32_t i = q0 + r * 256u + tid;
       uint32_t dst = count_finish(base[r], heads[r]);
-      if (loc[r] != kInvalidKey) dst = atomicAdd(&s_cnt[loc[r]], 1u);
-      if ((loc[r] != kInvalidKey || g[r] != kInvalidKey) && dst < I.cap) perm[dst] = i;
+      if (loc[r] != kInvalidKey) dst = lds[r];
+      if ((loc[r] != kInvalidKey || g[r] != kInvalidKey) && dst < cap) perm[dst] = i;
     }
   };
   // slots after the tile-ordered ones (or all of them before the first
@@ -505,7 +520,7 @@ __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip
     for (uint32_t r = 0; r < kScatterRounds; ++r) {
       const uint32_t i = q0 + r * 256u + tid;
       const uint32_t dst = count_finish(base[r], heads[r]);
-      if (key[r] != kInvalidKey && dst < I.cap) perm[dst] = i;
+      if (key[r] != kInvalidKey && dst < cap) perm[dst] = i;
     }
   };
   const uint32_t passes = S ? (I.ntiles + K - 1u) / K : 0u;
@@ -517,7 +532,10 @@ __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip
     __syncthreads();
     const uint32_t p0 = s_tp[0], p1 = max(s_tp[K], p0);
     uint32_t t = 0;
-    for (uint32_t q0 = p0; q0 < p1; q0 += 256u * kScatterRounds) place_tiled(q0, p1, c0, K, t);
+    uint32_t kk[kScatterRounds];
+    if (p0 < p1) load_keys(p0, p1, kk);
+    __builtin_amdgcn_s_waitcnt(0);  // the loop starts with no load pending, so its top waits for none
+    for (uint32_t q0 = p0; q0 < p1; q0 += 256u * kScatterRounds) place_tiled(q0, p1, c0, K, t, kk);
     __syncthreads();  // s_cnt and s_tp are set again for the next pass
   }
   for (uint32_t q0 = S + blockIdx.x * 256u * kScatterRounds; q0 < n; q0 += gridDim.x * 256u * kScatterRounds)
