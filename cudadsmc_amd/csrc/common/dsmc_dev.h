@@ -215,6 +215,14 @@ __device__ __forceinline__ uint32_t gatomic_add(gptr<uint32_t> p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Waits for every vector memory operation of the wave (s_waitcnt vmcnt(0);
+// expcnt and lgkmcnt not waited for). The compiler counts loads issued under
+// a branch as possibly not issued, so its own wait for the first use of such
+// a load also waits for every store issued after it; one wait here, where
+// the loads of a batch are all needed anyway, leaves no load pending for the
+// stores that follow.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Walk of workgroup vw (of a grid of G) over the positions of the tile
 // order. The hardware deals workgroups out to the 8 XCDs round-robin (XCD =
 // vw mod 8). With G a multiple of 8, the order is cut into patches of

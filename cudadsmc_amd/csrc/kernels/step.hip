@@ -833,16 +833,20 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
     const uint32_t jn = tile_queue_pos(dyn ? *s_pop : e_static, qx, per_xcd);
     tile_n = jn < I.ntiles ? (I.tile_order ? gmem(I.tile_order)[jn] : jn) : kInvalidKey;
   }
-  // the collision lane's cell state, loaded now so that its latency overlaps
-  // the velocity gathers
+  // the collision lane's cell state, requested with the first velocity
+  // gathers so that the latencies overlap (lanes without a gather request it
+  // before the loop)
   float2 nt_pf = make_float2(0.0f, 0.0f);
   float frac_pf = 1.0f;
   float4 st_pf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (tid < nc && s_start[tid + 1] - s_start[tid] >= 2u) {
-    nt_pf = gload(gmem<const float2>(I.ntc) + c0 + tid);
-    if (MODE == kModeReference) st_pf = gload(gmem<const float4>(I.cstat) + c0 + tid);
-    else if (I.fluid_frac) frac_pf = gmem(I.fluid_frac)[c0 + tid];
-  }
+  auto load_cell_state = [&]() {
+    if (tid < nc && s_start[tid + 1] - s_start[tid] >= 2u) {
+      nt_pf = gload(gmem<const float2>(I.ntc) + c0 + tid);
+      if (MODE == kModeReference) st_pf = gload(gmem<const float4>(I.cstat) + c0 + tid);
+      else if (I.fluid_frac) frac_pf = gmem(I.fluid_frac)[c0 + tid];
+    }
+  };
+  if (tid >= np) load_cell_state();
   // (3) gather velocities in sorted order; the loads of kMoveBatch positions
   // are issued before any store so that their latencies overlap
   for (uint32_t q0 = tid; q0 < np; q0 += kMoveBatch * kTileThreads) {
@@ -858,6 +862,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
         gz[k] = ivz[src];
       }
     }
+    if (q0 == tid) load_cell_state();
 #pragma unroll
     for (uint32_t k = 0; k < kMoveBatch; ++k) {
       const uint32_t q = q0 + k * kTileThreads;
@@ -869,6 +874,10 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
     }
   }
   __syncthreads();
+  // Every load issued so far has arrived (the velocity gather waited for
+  // them); saying so here keeps the compiler from waiting for the next
+  // tile's loads below at the first use of an older one in the NTC phase.
+  wait_vmem();
   // the next tile's cell offsets and near flags, stored to the other LDS
   // buffer at the end of this phase: the next tile starts without a load
   uint32_t nx_start = 0u, nx_end = 0u, ncn = 0u;
@@ -906,35 +915,52 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
         gstore(gmem(I.ntc) + c, nt);
       }
     }
-  } else {
-    float2 nt = nt_pf;
-    tile_ntc_rounds(I, c0, nc, p0, s_start, vx, vy, vz, nt, frac_pf, stepno, ntl, cacc);
-    if (tid < nc && s_start[tid + 1] - s_start[tid] >= 2u) gstore(gmem(I.ntc) + c0 + tid, nt);
+  }
+  float2 ntc_state = nt_pf;
+  if (MODE == kModePhysics) {
+    // frac_pf is first used here: without this marker the compiler applies
+    // the NTC's clamp of it right after its load, whose wait then comes
+    // before the velocity gather instead of after it
+    asm volatile("" : "+v"(frac_pf));
+    tile_ntc_rounds(I, c0, nc, p0, s_start, vx, vy, vz, ntc_state, frac_pf, stepno, ntl, cacc);
   }
   if (tile_n != kInvalidKey) {
     sn_start[tid] = nx_start;
     if (tid == 0) sn_start[ncn] = nx_end;
     if (MODE == kModePhysics) sn_nearc[tid] = nx_near;
   }
+  if (MODE == kModePhysics) {  // after the next tile's offsets, so that their wait is not a wait for this store
+    if (tid < nc && s_start[tid + 1] - s_start[tid] >= 2u) gstore(gmem(I.ntc) + c0 + tid, ntc_state);
+  }
   // per-cell counts of the particles that stay in the tile (the NTC is done
   // with the offsets that share this LDS)
   ntl.off[tid] = 0u;
   if (tid == 0) gmem(I.tile_p0)[tile] = p0;
   __syncthreads();
-  // the next tile's first permutation entries (its offsets are in LDS now)
-  if (MODE == kModePhysics && tile_n != kInvalidKey) {
-    const uint32_t p0n = sn_start[0];
-    const uint32_t npn = sn_start[ncn] - p0n;
+  // the next tile's first permutation entries (its offsets are in LDS now),
+  // requested with the first batch of position gathers and waited for with
+  // them, so that no later wait (the next tile's first use of pf comes after
+  // this tile's stores) has to wait for the stores
+  auto load_pf = [&]() {
+    if (MODE == kModePhysics && tile_n != kInvalidKey) {
+      const uint32_t p0n = sn_start[0];
+      const uint32_t npn = sn_start[ncn] - p0n;
 #pragma unroll
-    for (uint32_t k = 0; k < kPermPrefetch; ++k) {
-      const uint32_t q = tid + k * kTileThreads;
-      pf[k] = (npn <= kLdsCap && q < npn) ? gmem(I.perm)[p0n + q] : 0u;
+      for (uint32_t k = 0; k < kPermPrefetch; ++k) {
+        const uint32_t q = tid + k * kTileThreads;
+        pf[k] = (npn <= kLdsCap && q < npn) ? gmem(I.perm)[p0n + q] : 0u;
+      }
     }
-  }
+  };
   // (5) move, walls, box faces, next key; rounds keep waves converged for
   // the aggregated counting
   const uint32_t rounds = (np + kTileThreads - 1u) / kTileThreads;
+  if (rounds == 0u) {
+    load_pf();
+    wait_vmem();
+  }
   for (uint32_t r0 = 0; r0 < rounds; r0 += kMoveBatch) {
+    if (r0 == 0u) load_pf();
     // positions of up to kMoveBatch rounds in flight before the first store
     float bx[kMoveBatch], by[kMoveBatch], bz[kMoveBatch];
     uint32_t bsrc[kMoveBatch];
@@ -951,6 +977,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
         bz[k] = ipz[src];
       }
     }
+    wait_vmem();
 #pragma unroll
     for (uint32_t k = 0; k < kMoveBatch; ++k) {
     if (r0 + k >= rounds) break;  // uniform over the workgroup
@@ -1125,6 +1152,7 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
       pf[k] = (np <= kLdsCap && q < np) ? gmem(I.perm)[p0 + q] : 0u;
     }
   }
+  wait_vmem();  // the first tile needs them now; later tiles find theirs arrived
   for (uint32_t b = 0; tile != kInvalidKey; b ^= 1u) {
     uint32_t tile_n = kInvalidKey;  // set by tile_body
     e += e_stride;                  // the next element of the fixed sequence
