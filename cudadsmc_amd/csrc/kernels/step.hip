@@ -430,7 +430,7 @@ __global__ void __launch_bounds__(kScanThreads) k_bscan_apply(const InstDesc* __
 // its cell's segment. The tile kernel stored the particles of each tile in
 // one run of slots (tile_p0) and counted per cell how many of them stayed
 // in the tile's own cells (local_cnt); the scan set cursor = cell_start +
-// local_cnt. A workgroup takes kScatterSpan consecutive cells (whole
+// local_cnt. A workgroup takes up to kScatterSpan consecutive cells (whole
 // tiles) and the slots of their tiles: the particles that stayed in their
 // tile are ranked with LDS atomics into the first local_cnt places of their
 // cells; the others (they left their tile, went through the wall pass, or
@@ -459,7 +459,9 @@ __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip
   // end of the tile-ordered slots; instances with small tiles place every
   // particle through the cursors (their tile kernel counts none as stayed)
   const uint32_t S = I.tile_dyn ? min(tp0[I.ntiles], n) : 0u;
-  const uint32_t K = kScatterSpan / I.tile_cells;  // tiles per pass
+  // tiles per pass: up to kScatterSpan cells, fewer when the instance has
+  // too few tiles to give every workgroup a pass
+  const uint32_t K = max(1u, min(kScatterSpan / tile_cells, I.ntiles / gridDim.x));
   // Slots of the pass's tiles [t0, t0 + K) (runs starting at s_tp[]), in
   // rounds of 256 slots from q0 (below q1). Each lane keeps the tile of its
   // last slot in t (its slots only grow). A particle that stayed in its
@@ -527,7 +529,7 @@ __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip
   for (uint32_t g = blockIdx.x; g < passes; g += gridDim.x) {  // uniform over the workgroup
     const uint32_t t0 = g * K;
     const uint32_t c0 = t0 * I.tile_cells;
-    for (uint32_t c = tid; c < kScatterSpan; c += 256u) s_cnt[c] = c0 + c < I.ncells ? start[c0 + c] : 0u;
+    for (uint32_t c = tid; c < K * tile_cells; c += 256u) s_cnt[c] = c0 + c < I.ncells ? start[c0 + c] : 0u;
     if (tid <= K) s_tp[tid] = t0 + tid <= I.ntiles ? min(tp0[t0 + tid], S) : S;
     __syncthreads();
     const uint32_t p0 = s_tp[0], p1 = max(s_tp[K], p0);

@@ -331,7 +331,9 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
   // four per resident workgroup (c1, 64x32x32, 100k particles: 0.123 ms per
   // step with 64-cell tiles, 0.132 with 256, 0.149 with 32; c2, 128x64x64,
   // 1M particles: 0.155 ms with 128-cell tiles, 0.171 with 256, 0.182 with
-  // 64).
+  // 64), and to 32 cells where such a tile still expects 512 particles (the
+  // reference's 32x32x32 benchmark case with 17 per cell: 0.135 ms per step
+  // with 32-cell tiles, 0.143 with 64).
   {
     double ppc_exp = phys ? (s.target_particles ? (double)s.target_particles / C : (double)s.density *
                              I.cell_volume / (double)s.fnum)
@@ -339,6 +341,7 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
     uint32_t tc = kTileCells;
     while (tc > 32u && tc * ppc_exp > 0.5 * kLdsCap) tc >>= 1;
     while (tc > 64u && (C + tc - 1) / tc < 4096u) tc >>= 1;
+    if (tc == 64u && (C + tc - 1) / tc < 4096u && 32.0 * ppc_exp >= 512.0) tc = 32u;
     I.tile_cells = tc;
     I.ntiles = (C + tc - 1) / tc;
     h->ppc_exp = ppc_exp;
