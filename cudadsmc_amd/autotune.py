@@ -167,7 +167,7 @@ def main(argv=None) -> None:
                 print("c3", json.dumps(rtab[-1]), flush=True)
             refine = min(rtab, key=lambda r: r["ms_per_step"])["cg_refine"]
             res = tune_solver(lambda ns: make(ns, refine), streams=(262144, 524288, 1048576, 2097152),
-                              scatter=(1024, 2048, 4096, 8192), warmup=600, steps=40)
+                              scatter=(2048, 4096, 8192, 16384, 32768), warmup=600, steps=40)
             res["cg_refine"] = refine
             res["table"] = rtab + res["table"]
         else:
