@@ -30,6 +30,7 @@ __host__ __device__ inline uint32_t key_cell(uint32_t k) { return k >= kDeferred
 constexpr uint32_t kTileCells = 256;    // most cells per tile (one lane per cell)
 constexpr uint32_t kTileThreads = 256;  // threads of the tile kernel
 constexpr uint32_t kLdsCap = 2048;      // particles staged in LDS per tile
+constexpr uint32_t kMaxCap = 1u << 30;   // particles per instance: 4-byte arrays stay below 4 GiB
 constexpr uint32_t kMtWords = 20;       // 19 window words + 1 spare (zero)
 constexpr uint32_t kRefMtStreams = 65536;  // the reference's MT_RNG_COUNT (MersenneTwister.h:31)
 constexpr int kMtN = 19, kMtM = 9;
@@ -178,6 +179,16 @@ using gptr = __attribute__((address_space(1))) T*;
 template <class T>
 __device__ __forceinline__ gptr<T> gmem(T* p) {
   return (gptr<T>)p;
+}
+
+// Element i of a global array addressed by a 32-bit byte offset from its
+// base (every particle and cell array is below 4 GiB: the capacity is at
+// most kMaxCap, the cells at most 16M). With a workgroup-uniform base the access compiles to the
+// scalar-base form (global_load vdst, voffset, sbase) instead of a 64-bit
+// address per lane, which costs two VALU instructions per access.
+template <class T>
+__device__ __forceinline__ gptr<T> gelem(gptr<T> base, uint32_t i) {
+  return (gptr<T>)((gptr<const char>)base + (uint32_t)(i * (uint32_t)sizeof(T)));
 }
 
 // HIP's vector structs cannot be copied out of a global-space reference: go

@@ -62,6 +62,8 @@ __device__ __forceinline__ void mt_store(const InstDesc& I, uint32_t s, const Mt
 // count_finish the lane's value.
 __device__ __forceinline__ uint32_t add_returning(gptr<uint32_t> p, uint32_t v) { return gatomic_add(p, v); }
 __device__ __forceinline__ uint32_t add_returning(uint32_t* p, uint32_t v) { return atomicAdd(p, v); }  // LDS
+__device__ __forceinline__ gptr<uint32_t> cnt_elem(gptr<uint32_t> p, uint32_t i) { return gelem(p, i); }
+__device__ __forceinline__ uint32_t* cnt_elem(uint32_t* p, uint32_t i) { return p + i; }
 
 template <class P>
 __device__ __forceinline__ uint32_t count_issue(P cnt, uint32_t key, uint64_t& heads) {
@@ -74,7 +76,7 @@ __device__ __forceinline__ uint32_t count_issue(P cnt, uint32_t key, uint64_t& h
     const uint64_t below_incl = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
     const uint64_t above = heads & ~below_incl;
     const uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
-    base = add_returning(cnt + key, next - lane);
+    base = add_returning(cnt_elem(cnt, key), next - lane);
   }
   return base;
 }
@@ -97,7 +99,7 @@ __device__ __forceinline__ void count_keys(gptr<uint32_t> cnt, uint32_t key) {
     uint64_t above = heads & ~((2ull << lane) - 1ull);
     if (lane == 63) above = 0;
     uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
-    gatomic_add(cnt + key, next - lane);
+    gatomic_add(gelem(cnt, key), next - lane);
   }
 }
 
@@ -476,7 +478,7 @@ __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip
 #pragma unroll
     for (uint32_t r = 0; r < kScatterRounds; ++r) {
       const uint32_t i = q0 + r * 256u + tid;
-      kk[r] = i < q1 ? keys[i] : kInvalidKey;
+      kk[r] = i < q1 ? *gelem(keys, i) : kInvalidKey;
     }
   };
   auto place_tiled = [&](uint32_t q0, uint32_t q1, uint32_t c0, uint32_t K, uint32_t& t,
@@ -503,7 +505,7 @@ __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip
       const uint32_t i = q0 + r * 256u + tid;
       uint32_t dst = count_finish(base[r], heads[r]);
       if (loc[r] != kInvalidKey) dst = lds[r];
-      if ((loc[r] != kInvalidKey || g[r] != kInvalidKey) && dst < cap) perm[dst] = i;
+      if ((loc[r] != kInvalidKey || g[r] != kInvalidKey) && dst < cap) *gelem(perm, dst) = i;
     }
   };
   // slots after the tile-ordered ones (or all of them before the first
@@ -515,14 +517,14 @@ __global__ void __launch_bounds__(256) k_scatter(const InstDesc* __restrict__ Ip
 #pragma unroll
     for (uint32_t r = 0; r < kScatterRounds; ++r) {
       const uint32_t i = q0 + r * 256u + tid;
-      key[r] = i < q1 ? key_cell(keys[i]) : kInvalidKey;
+      key[r] = i < q1 ? key_cell(*gelem(keys, i)) : kInvalidKey;
       base[r] = count_issue(cursor, key[r], heads[r]);
     }
 #pragma unroll
     for (uint32_t r = 0; r < kScatterRounds; ++r) {
       const uint32_t i = q0 + r * 256u + tid;
       const uint32_t dst = count_finish(base[r], heads[r]);
-      if (key[r] != kInvalidKey && dst < cap) perm[dst] = i;
+      if (key[r] != kInvalidKey && dst < cap) *gelem(perm, dst) = i;
     }
   };
   const uint32_t passes = S ? (I.ntiles + K - 1u) / K : 0u;
@@ -802,9 +804,9 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
 #pragma unroll
       for (uint32_t k = 0; k < kPermPrefetch; ++k)
         if (tid + k * kTileThreads < np) raw[tid + k * kTileThreads] = pf[k];
-      for (uint32_t q = tid + kPermPrefetch * kTileThreads; q < np; q += kTileThreads) raw[q] = gmem(I.perm)[p0 + q];
+      for (uint32_t q = tid + kPermPrefetch * kTileThreads; q < np; q += kTileThreads) raw[q] = *gelem(gmem(I.perm), p0 + q);
     } else {
-      for (uint32_t q = tid; q < np; q += kTileThreads) raw[q] = gmem(I.perm)[p0 + q];
+      for (uint32_t q = tid; q < np; q += kTileThreads) raw[q] = *gelem(gmem(I.perm), p0 + q);
     }
   }
   if (tid < nc) {
@@ -859,9 +861,9 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
       gx[k] = gy[k] = gz[k] = 0.0f;
       if (q < np) {
         const uint32_t src = srt[q];
-        gx[k] = ivx[src];
-        gy[k] = ivy[src];
-        gz[k] = ivz[src];
+        gx[k] = *gelem(ivx, src);
+        gy[k] = *gelem(ivy, src);
+        gz[k] = *gelem(ivz, src);
       }
     }
     if (q0 == tid) load_cell_state();
@@ -950,7 +952,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
 #pragma unroll
       for (uint32_t k = 0; k < kPermPrefetch; ++k) {
         const uint32_t q = tid + k * kTileThreads;
-        pf[k] = (npn <= kLdsCap && q < npn) ? gmem(I.perm)[p0n + q] : 0u;
+        pf[k] = (npn <= kLdsCap && q < npn) ? *gelem(gmem(I.perm), p0n + q) : 0u;
       }
     }
   };
@@ -974,9 +976,9 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
       if (q < np) {
         const uint32_t src = srt[q];
         bsrc[k] = src;
-        bx[k] = ipx[src];
-        by[k] = ipy[src];
-        bz[k] = ipz[src];
+        bx[k] = *gelem(ipx, src);
+        by[k] = *gelem(ipy, src);
+        bz[k] = *gelem(ipz, src);
       }
     }
     wait_vmem();
@@ -1025,7 +1027,7 @@ __device__ __forceinline__ void tile_body(const InstDesc& I, uint32_t par, uint3
         vz[q] = uz;
       }
       if (flags & kFlagHits) gmem(I.hits)[p0 + q] = hits;
-      if (flags & kFlagKeepPerm) gmem(I.perm)[p0 + q] = src;
+      if (flags & kFlagKeepPerm) *gelem(gmem(I.perm), p0 + q) = src;
     }
     count_keys(gmem(I.count), key == kDeferredKey ? kInvalidKey : key);
     if (I.tile_dyn && key - c0 < nc) atomicAdd(&ntl.off[key - c0], 1u);  // stayed in the tile's cells
@@ -1151,7 +1153,7 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_tile(const InstDesc* __rest
 #pragma unroll
     for (uint32_t k = 0; k < kPermPrefetch; ++k) {
       const uint32_t q = tid + k * kTileThreads;
-      pf[k] = (np <= kLdsCap && q < np) ? gmem(I.perm)[p0 + q] : 0u;
+      pf[k] = (np <= kLdsCap && q < np) ? *gelem(gmem(I.perm), p0 + q) : 0u;
     }
   }
   wait_vmem();  // the first tile needs them now; later tiles find theirs arrived

@@ -325,6 +325,8 @@ int dsmc_init(dsmc_solver* h, const dsmc_settings* sp) {
   } else {
     cap = s.capacity ? std::max(s.capacity, s.num_particles) : std::max<uint32_t>(s.num_particles, 1u);
   }
+  // the kernels address particle arrays by 32-bit byte offsets
+  if (cap > kMaxCap) return h->fail(DSMC_ERR_CAPACITY, "capacity %u above %u particles", cap, kMaxCap);
   // Cells per tile: keep the expected particles of a tile near half the LDS
   // capacity so that few tiles take the global-memory path; on small grids,
   // halve the tiles (down to 64 cells) until there are 4096 of them, about

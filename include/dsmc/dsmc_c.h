@@ -91,7 +91,7 @@ typedef struct dsmc_settings {
   uint32_t target_particles; /* PHYSICS: if > 0, fnum = density*V_fluid/target */
   uint32_t seed;             /* first MT seed; stream s is seeded with seed + s */
   uint32_t sample_every;     /* sampling cadence (0: every 100th step as the reference) */
-  uint32_t capacity;         /* particle buffer capacity (0: automatic) */
+  uint32_t capacity;         /* particle buffer capacity (0: automatic; at most 2^30) */
   uint32_t n_streams;        /* RNG streams (0: automatic, multiple of 256) */
   char rng_param_file[256];  /* optional MT parameter rows (16-byte records:
                                 matrix_a, mask_b, mask_c, unused), e.g. the
