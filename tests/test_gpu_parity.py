@@ -303,3 +303,22 @@ def test_gpu_graph_replay_matches_eager():
     for k in ("pos", "vel", "key"):
         assert np.array_equal(p0[k], p1[k]), k
     assert np.array_equal(c0, c1) and np.array_equal(r0, r1) and n0 == n1
+
+
+def test_gpu_capacity_limit():
+    # the kernels address particle arrays by 32-bit byte offsets, so a
+    # capacity above 2^30 particles is refused at init, before any allocation
+    note("capacity limit")
+    s = small_physics_settings()
+    s.capacity = (1 << 30) + 1
+    sol = dm.Solver()
+    with pytest.raises(dm.DsmcError, match="capacity"):
+        sol.Init(s)
+    sol.close()
+    s.capacity = 1 << 20
+    sol = dm.Solver()
+    sol.Init(s)
+    sol.SetModel(dm.icosphere((0.07, 0.05, 0.05), 0.02, 2), grid=(8, 8, 8))
+    assert sol.InitSystemState((1500.0, 0.0, 0.0)) > 0
+    sol.DoBenchmark(2, s.dt)
+    sol.close()
