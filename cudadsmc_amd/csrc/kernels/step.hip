@@ -52,21 +52,14 @@ __device__ __forceinline__ void mt_store(const InstDesc& I, uint32_t s, const Mt
   for (int w = 0; w < kMtN; ++w) gmem(I.mt_state)[(uint32_t)w * I.n_streams + s] = m.st[w];
 }
 
-// Wave-aggregated atomicAdd(&cnt[key], 1) for keys of consecutive lanes
-// (cnt global or in LDS), returning each lane's count before its own
-// increment: one atomic per run
+// Wave-aggregated atomicAdd(&cnt[key], 1) for keys of consecutive lanes,
+// returning each lane's count before its own increment: one atomic per run
 // of equal keys, whose lanes take consecutive values. Invalid keys are
 // skipped (their result is meaningless). Split in two so that a caller can
 // issue the atomics of several rounds before waiting for any: count_issue
 // returns the run head's old count (0 elsewhere) and the wave's run heads,
 // count_finish the lane's value.
-__device__ __forceinline__ uint32_t add_returning(gptr<uint32_t> p, uint32_t v) { return gatomic_add(p, v); }
-__device__ __forceinline__ uint32_t add_returning(uint32_t* p, uint32_t v) { return atomicAdd(p, v); }  // LDS
-__device__ __forceinline__ gptr<uint32_t> cnt_elem(gptr<uint32_t> p, uint32_t i) { return gelem(p, i); }
-__device__ __forceinline__ uint32_t* cnt_elem(uint32_t* p, uint32_t i) { return p + i; }
-
-template <class P>
-__device__ __forceinline__ uint32_t count_issue(P cnt, uint32_t key, uint64_t& heads) {
+__device__ __forceinline__ uint32_t count_issue(gptr<uint32_t> cnt, uint32_t key, uint64_t& heads) {
   const uint32_t lane = lane_id();
   const uint32_t prev = __shfl_up(key, 1);
   const bool head = (lane == 0) || (key != prev);
@@ -76,7 +69,7 @@ __device__ __forceinline__ uint32_t count_issue(P cnt, uint32_t key, uint64_t& h
     const uint64_t below_incl = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
     const uint64_t above = heads & ~below_incl;
     const uint32_t next = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : 64u;
-    base = add_returning(cnt_elem(cnt, key), next - lane);
+    base = gatomic_add(gelem(cnt, key), next - lane);
   }
   return base;
 }
