@@ -1022,7 +1022,7 @@ int format_statistics(dsmc_solver* h, const dsmc_sim_state& st, std::string& tex
       << "\nCollision ratio: " << col / sel << "\n";
   out << "Simulation time: " << st.sim_time << " Running time: " << st.real_time << "\n";
   out << "Cell N  ||  SampNum  ||  Dens  ||          Velocity          ||  Temperature\n";
-  double tot = 0, denn = 0, avg_denn = 0;
+  double tot = 0, denn = 0;
   const float cell_volume = h->s.cell_size[0] * h->s.cell_size[1] * h->s.cell_size[2];
   const float kb = kBoltzRef;
   for (uint32_t i = 0; i < I.ncells; ++i) {
@@ -1030,7 +1030,6 @@ int format_statistics(dsmc_solver* h, const dsmc_sim_state& st, std::string& tex
     float a = h->s.fnum / (cell_volume * (float)st.samples_count);
     const float n = (float)c.num_in_sample;
     denn = a * n;
-    avg_denn += denn;
     float vx = (float)c.xyz_sum[0] / n, vy = (float)c.xyz_sum[1] / n, vz = (float)c.xyz_sum[2] / n;
     float uu = vx * vx + vy * vy + vz * vz;
     float tt = h->s.molmass * ((float)c.vel_mag / n - uu) / (3.0f * kb);
@@ -1039,7 +1038,6 @@ int format_statistics(dsmc_solver* h, const dsmc_sim_state& st, std::string& tex
         << tt << "\n";
   }
   tot /= I.ncells;
-  avg_denn /= I.ncells;
   const float gv = (h->s.bmax[0] - h->s.bmin[0]) * (h->s.bmax[1] - h->s.bmin[1]) * (h->s.bmax[2] - h->s.bmin[2]);
   // the reference names these dens2 / diam2 but forms n*d twice; the product
   // n^2 d^2 is the hard-sphere collision-rate factor
@@ -1049,6 +1047,8 @@ int format_statistics(dsmc_solver* h, const dsmc_sim_state& st, std::string& tex
   float tColl = (float)(2.0f * st.sim_time * dens2 * gv * kPiRef * diam2 * std::pow(tot / Tref, 0.25) *
                         sqrtf((float)(kb * Tref / (kPiRef * molmas))) / h->s.fnum);
   out << "\n\nExpected collisions: " << tColl << " Ratio: " << tColl / col << "\n";
+  // the reference prints the last cell's density under "Average density"
+  // (dsmc_base.cu:162); kept for output compatibility
   out << "Average temperature: " << tot << " Reference temperature is: " << h->s.temp
       << "\nAverage density: " << denn << " Reference density: " << h->s.density << "\n\n" << std::endl;
   text = out.str();
