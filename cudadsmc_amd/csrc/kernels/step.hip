@@ -636,7 +636,7 @@ constexpr uint32_t kMoveBatch = 6;  // rounds of 256 positions whose loads are i
 // PHYSICS: rounds of 256 permutation entries of the next tile loaded into
 // registers during this tile's move (the next tile's first phase then only
 // stores them to LDS)
-constexpr uint32_t kPermPrefetch = 4;
+constexpr uint32_t kPermPrefetch = 6;
 
 // LDS of the PHYSICS NTC rounds.
 struct NtcLds {
