@@ -305,6 +305,34 @@ def test_gpu_graph_replay_matches_eager():
     assert np.array_equal(c0, c1) and np.array_equal(r0, r1) and n0 == n1
 
 
+@pytest.mark.parametrize("grid", [64, 256, 512, 4096])
+def test_gpu_scatter_pass_sizes_bit_exact(grid):
+    # 1024 tiles of 64 cells with about 640 particles each (the tile queues
+    # and the tile-local scatter): the scatter takes 16, 4, 2 and 1 tiles per
+    # workgroup pass at these grids; every pass size gives the oracle's state
+    note(f"scatter passes, grid {grid}")
+    s = small_physics_settings(dims=(64, 32, 32), n_target=655360)
+    sol = dm.Solver()
+    sol.Init(s)
+    assert sol.layout()["tile_cells"] == 64
+    sol.set_scatter_grid(grid)
+    mesh = dm.icosphere((0.07, 0.05, 0.05), 0.02, 2)
+    sol.SetModel(mesh, grid=(8, 8, 8))
+    o = _mirror(sol, mesh)
+    sol.InitSystemState(tuple(s.stream_velocity))
+    o.load(tuple(s.stream_velocity))
+    sol.set_debug(_capi.FLAG_KEEP_PERM)
+    for k in range(4):
+        sol.RunBenchmarkSimulationStep(s.dt)
+        o.step(sample=False)
+        tag = f"grid {grid} step {k + 1}"
+        assert np.array_equal(sol.cell_start(), o.cell_start()), f"{tag}: cell offsets"
+        npart = int(o.cell_start()[-1])
+        assert np.array_equal(sol.perm(npart), o.perm(npart)), f"{tag}: permutation"
+        _compare_state(sol, o, tag)
+    sol.close()
+
+
 def test_gpu_capacity_limit():
     # the kernels address particle arrays by 32-bit byte offsets, so a
     # capacity above 2^30 particles is refused at init, before any allocation
