@@ -112,20 +112,20 @@ __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
 __device__ __forceinline__ bool ref_collide_geometry(const InstDesc& I, float& px, float& py, float& pz,
                                                      float& vx, float& vy, float& vz, float& ox, float& oy,
                                                      float& oz, uint32_t cell) {
-  uint32_t s = I.cg_start[cell], e = I.cg_end[cell];
+  uint32_t s = gmem(I.cg_start)[cell], e = gmem(I.cg_end)[cell];
   float dx = vx, dy = vy, dz = vz;
   normalize3(dx, dy, dz);
   float ddx = px - ox, ddy = py - oy, ddz = pz - oz;
   float dist = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);  // distance4 (vectors.cu:101-106)
   if (dist < kGeomEps) return false;
   for (uint32_t i = s; i < e; ++i) {
-    uint32_t tri = I.cg_tri[i];
-    float4 a = I.tv0[tri], b = I.tv1[tri], c = I.tv2[tri];
+    uint32_t tri = gmem(I.cg_tri)[i];
+    float4 a = gload(gmem(I.tv0) + tri), b = gload(gmem(I.tv1) + tri), c = gload(gmem(I.tv2) + tri);
     float t, u, v;
     if (intersect_tri(ox, oy, oz, dx, dy, dz, a, b, c, t, u, v)) {
       if (t >= 0.0f && t <= dist) {
         float uv = 1.0f - u - v;
-        float4 n = I.tn[tri];
+        float4 n = gload(gmem(I.tn) + tri);
         float wx = n.x * kModelWidth, wy = n.y * kModelWidth, wz = n.z * kModelWidth;
         float hx = a.x * uv + b.x * u + c.x * v;
         float hy = a.y * uv + b.y * u + c.y * v;
@@ -216,10 +216,10 @@ __device__ __forceinline__ uint32_t move_reference(const InstDesc& I, float& px,
       uint32_t st = cg_cell(I, ox, oy, oz);
       uint32_t en = cg_cell(I, nx, ny, nz);
       if (st != kInvalidKey) {
-        if (I.cg_flag[st] != 0u) coll = ref_collide_geometry(I, nx, ny, nz, vx, vy, vz, ox, oy, oz, st);
+        if (gmem(I.cg_flag)[st] != 0u) coll = ref_collide_geometry(I, nx, ny, nz, vx, vy, vz, ox, oy, oz, st);
       }
       if (!coll && st != en && en != kInvalidKey) {
-        if (I.cg_flag[en] != 0u) coll = ref_collide_geometry(I, nx, ny, nz, vx, vy, vz, ox, oy, oz, en);
+        if (gmem(I.cg_flag)[en] != 0u) coll = ref_collide_geometry(I, nx, ny, nz, vx, vy, vz, ox, oy, oz, en);
       }
       if (coll) ++hits;
     } while (coll && i++ < kCollisionLimit);
@@ -249,8 +249,8 @@ __device__ __forceinline__ bool ref_collide_wave(const InstDesc& I, bool want, f
   const float ddx = px - ox, ddy = py - oy, ddz = pz - oz;
   const float dist = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);  // distance4 (vectors.cu:101-106)
   if (want && !(dist < kGeomEps)) {
-    s = I.cg_start[cell];
-    n = I.cg_end[cell] - s;
+    s = gmem(I.cg_start)[cell];
+    n = gmem(I.cg_end)[cell] - s;
   }
   uint32_t incl = n;
 #pragma unroll
@@ -277,10 +277,10 @@ __device__ __forceinline__ bool ref_collide_wave(const InstDesc& I, bool want, f
     const float ex = __shfl(dx, (int)o), ey = __shfl(dy, (int)o), ez = __shfl(dz, (int)o);
     const float qd = __shfl(dist, (int)o);
     if (k0 + lane < total) {
-      const uint32_t tri = I.cg_tri[e];
+      const uint32_t tri = gmem(I.cg_tri)[e];
       float t, u, v;
-      if (intersect_tri(qx, qy, qz, ex, ey, ez, I.tv0[tri], I.tv1[tri], I.tv2[tri], t, u, v) && t >= 0.0f &&
-          t <= qd)
+      const float4 a = gload(gmem(I.tv0) + tri), b = gload(gmem(I.tv1) + tri), c = gload(gmem(I.tv2) + tri);
+      if (intersect_tri(qx, qy, qz, ex, ey, ez, a, b, c, t, u, v) && t >= 0.0f && t <= qd)
         atomicMin(&first[o], e);
     }
   }
@@ -289,12 +289,12 @@ __device__ __forceinline__ bool ref_collide_wave(const InstDesc& I, bool want, f
   if (!n) return false;
   const uint32_t e = first[lane];
   if (e == 0xFFFFFFFFu) return false;
-  const uint32_t tri = I.cg_tri[e];
-  const float4 a = I.tv0[tri], b = I.tv1[tri], c = I.tv2[tri];
+  const uint32_t tri = gmem(I.cg_tri)[e];
+  const float4 a = gload(gmem(I.tv0) + tri), b = gload(gmem(I.tv1) + tri), c = gload(gmem(I.tv2) + tri);
   float t, u, v;
   intersect_tri(ox, oy, oz, dx, dy, dz, a, b, c, t, u, v);
   const float uv = 1.0f - u - v;
-  const float4 nrm = I.tn[tri];
+  const float4 nrm = gload(gmem(I.tn) + tri);
   const float wx = nrm.x * kModelWidth, wy = nrm.y * kModelWidth, wz = nrm.z * kModelWidth;
   const float hx = a.x * uv + b.x * u + c.x * v;
   const float hy = a.y * uv + b.y * u + c.y * v;
@@ -333,10 +333,10 @@ __device__ __forceinline__ uint32_t move_reference_wave(const InstDesc& I, bool 
         st = cg_cell(I, ox, oy, oz);
         en = cg_cell(I, nx, ny, nz);
       }
-      const bool w1 = live && st != kInvalidKey && I.cg_flag[st] != 0u;
+      const bool w1 = live && st != kInvalidKey && gmem(I.cg_flag)[st] != 0u;
       const bool c1 = ref_collide_wave(I, w1, nx, ny, nz, vx, vy, vz, ox, oy, oz, st, first);
       if (w1) coll = c1;
-      const bool w2 = live && !coll && st != en && en != kInvalidKey && I.cg_flag[en] != 0u;
+      const bool w2 = live && !coll && st != en && en != kInvalidKey && gmem(I.cg_flag)[en] != 0u;
       const bool c2 = ref_collide_wave(I, w2, nx, ny, nz, vx, vy, vz, ox, oy, oz, en, first);
       if (w2) coll = c2;
       if (live) {
@@ -788,13 +788,14 @@ __device__ __forceinline__ bool inside_body(const InstDesc& I, float x, float y,
   if (!I.has_geom) return false;
   uint32_t c = cg_cell(I, x, y, z);
   if (c == kInvalidKey) return false;
-  uint32_t f = I.cg_flag[c];
+  uint32_t f = gmem(I.cg_flag)[c];
   if (f == kCgFilled) return true;
   if (f != kCgSurface) return false;
   float best = 3.0e38f, sd = 1.0f;
-  for (uint32_t i = I.cg_start[c]; i < I.cg_end[c]; ++i) {
-    uint32_t tri = I.cg_tri[i];
-    float4 a = I.tv0[tri], b = I.tv1[tri], cc = I.tv2[tri], n = I.tn[tri];
+  for (uint32_t i = gmem(I.cg_start)[c]; i < gmem(I.cg_end)[c]; ++i) {
+    uint32_t tri = gmem(I.cg_tri)[i];
+    float4 a = gload(gmem(I.tv0) + tri), b = gload(gmem(I.tv1) + tri), cc = gload(gmem(I.tv2) + tri);
+    float4 n = gload(gmem(I.tn) + tri);
     float cx = (a.x + b.x + cc.x) / 3.0f, cy = (a.y + b.y + cc.y) / 3.0f, cz = (a.z + b.z + cc.z) / 3.0f;
     float rx = x - cx, ry = y - cy, rz = z - cz;
     float d2 = rx * rx + ry * ry + rz * rz;
